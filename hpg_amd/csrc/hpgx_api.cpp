@@ -1,0 +1,1047 @@
+// C ABI of the MI355X gridding device (include/hpgx.h): state management,
+// argument validation, device memory, hipFFT, and the launch sequence of the
+// kernels in hpgx_kernels.hip.
+//
+// One hpgx_state_s corresponds to one hpg::runtime::StateT<D>
+// (src/hpg_runtime.hpp:720-1750): it owns the grid, weights, model, CF and
+// per-call device buffers, and one HIP stream on which all its work is
+// queued in order.
+
+#include "hpgx.h"
+#include "hpgx_internal.hpp"
+
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+using hpgx::CallBuffers;
+using hpgx::DeviceCF;
+using hpgx::Geometry;
+using hpgx::GroupInfo;
+using hpgx::Mueller;
+
+namespace {
+
+constexpr int num_event_slots = 16;
+
+std::mutex g_create_mtx;
+std::string g_create_error;  // message of the last failed hpgx_create
+int g_device = -1;
+
+struct Plan {
+  hipfftHandle h = 0;
+  bool valid = false;
+};
+
+}  // namespace
+
+struct hpgx_state_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  unsigned max_added_tasks = 0;
+  size_t max_batch = 0;
+  unsigned gsize[4] = {0, 0, 0, 0};
+  double gscale[2] = {0.0, 0.0};
+  unsigned npol = 0;
+  int mode = HPGX_GRID_FP64;
+  Mueller mu{};
+  std::vector<int> mueller_rows, conj_mueller_rows;  // as given, rows x npol
+
+  void* grid = nullptr;       // double2 or float2, Layout::Left
+  double* weights = nullptr;  // [cube][mrow]
+  double2* model = nullptr;   // Layout::Left, optional
+  void* scratch = nullptr;    // grid-sized, shift / out-of-place FFT / readback
+
+  // convolution function
+  DeviceCF cf{};
+  std::vector<GroupInfo> groups;
+  size_t taps_capacity = 0;   // float2 elements
+  size_t wsum_capacity = 0;   // double2 elements
+  float2* staging = nullptr;
+  size_t staging_capacity = 0;
+
+  CallBuffers cb{};
+  size_t vis_capacity = 0;  // records
+  size_t nkeys = 0;
+
+  Plan plan_z2z, plan_c2c;
+  hipEvent_t events[num_event_slots] = {};
+  std::string err;
+
+  size_t grid_elems() const {
+    return size_t(gsize[0]) * gsize[1] * gsize[2] * gsize[3];
+  }
+  size_t grid_bytes() const {
+    return grid_elems() * (mode == HPGX_GRID_FP64 ? sizeof(double2) : sizeof(float2));
+  }
+  Geometry geometry() const {
+    Geometry g{};
+    g.nx = int(gsize[0]);
+    g.ny = int(gsize[1]);
+    g.nr = int(gsize[2]);
+    g.nc = int(gsize[3]);
+    g.scale[0] = gscale[0];
+    g.scale[1] = gscale[1];
+    g.os = cf.os;
+    g.F = cf.F;
+    g.nf = cf.nf;
+    g.ngroups = cf.ngroups;
+    g.ntx = (g.nx + hpgx::tile - 1) / hpgx::tile;
+    g.nty = (g.ny + hpgx::tile - 1) / hpgx::tile;
+    return g;
+  }
+};
+
+namespace {
+
+int fail(hpgx_state_t st, int code, const std::string& msg) {
+  if (st) st->err = msg;
+  return code;
+}
+
+int hip_fail(hpgx_state_t st, hipError_t e, const char* what) {
+  return fail(
+    st, HPGX_ERR_OTHER, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HPGX_HIP(st, call)                                 \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
+    if (e_ != hipSuccess) return hip_fail((st), e_, #call); \
+  } while (0)
+
+template <typename T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+template <typename T>
+hipError_t dev_alloc(T*& p, size_t bytes) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes ? bytes : 1);
+  p = static_cast<T*>(q);
+  return e;
+}
+
+size_t visdata_size(unsigned npol) {
+  switch (npol) {
+  case 1: return sizeof(hpgx_visdata1_t);
+  case 2: return sizeof(hpgx_visdata2_t);
+  case 3: return sizeof(hpgx_visdata3_t);
+  case 4: return sizeof(hpgx_visdata4_t);
+  default: return 0;
+  }
+}
+static_assert(sizeof(hpgx_visdata1_t) == 88, "hpg::VisData<1> layout");
+static_assert(sizeof(hpgx_visdata2_t) == 96, "hpg::VisData<2> layout");
+static_assert(sizeof(hpgx_visdata3_t) == 112, "hpg::VisData<3> layout");
+static_assert(sizeof(hpgx_visdata4_t) == 120, "hpg::VisData<4> layout");
+
+// CFArray::radii (src/hpg.hpp:687-694)
+void radii(const unsigned* ext, unsigned os, int& rx, int& ry) {
+  rx = int(((ext[0] - 2 * hpgx::cf_padding * os) / os) / 2);
+  ry = int(((ext[1] - 2 * hpgx::cf_padding * os) / os) / 2);
+}
+
+// device CF size in float2 elements for a CF shape (fine-offset slices)
+size_t cf_taps_for(unsigned os, unsigned ngroups, const unsigned* extents,
+                   size_t* wsum_elems = nullptr) {
+  const size_t F = (os + 1) / 2, nf = 2 * F + 1;
+  size_t taps = 0, ws = 0;
+  for (unsigned g = 0; g < ngroups; ++g) {
+    const unsigned* e = extents + 4 * g;
+    int rx, ry;
+    radii(e, os, rx, ry);
+    const size_t slices = size_t(e[2]) * e[3] * nf * nf;
+    taps += slices * size_t(2 * rx + 1) * size_t(2 * ry + 1);
+    ws += slices;
+  }
+  if (wsum_elems) *wsum_elems = ws;
+  return taps;
+}
+
+int ensure_cf_capacity(hpgx_state_t st, size_t taps, size_t ws) {
+  if (taps > st->taps_capacity) {
+    dev_free(st->cf.taps);
+    st->taps_capacity = 0;
+    HPGX_HIP(st, dev_alloc(st->cf.taps, taps * sizeof(float2)));
+    st->taps_capacity = taps;
+  }
+  if (ws > st->wsum_capacity) {
+    dev_free(st->cf.wsum);
+    st->wsum_capacity = 0;
+    HPGX_HIP(st, dev_alloc(st->cf.wsum, ws * sizeof(double2)));
+    st->wsum_capacity = ws;
+  }
+  if (!st->cf.groups)
+    HPGX_HIP(st, dev_alloc(st->cf.groups, hpgx::max_cf_groups * sizeof(GroupInfo)));
+  return HPGX_SUCCESS;
+}
+
+// per-call buffers sized for max_batch records and the current CF supports
+int ensure_call_buffers(hpgx_state_t st) {
+  CallBuffers& cb = st->cb;
+  if (!cb.vis) {
+    const size_t n = std::max<size_t>(st->max_batch, 1);
+    HPGX_HIP(st, dev_alloc(cb.vis, n * visdata_size(st->npol)));
+    HPGX_HIP(st, dev_alloc(cb.prep, n * sizeof(hpgx::PrepVis)));
+    HPGX_HIP(st, dev_alloc(cb.gvis, n * st->npol * sizeof(float2)));
+    HPGX_HIP(st, dev_alloc(cb.invalid_count, sizeof(unsigned long long)));
+    HPGX_HIP(st, hipMemsetAsync(cb.invalid_count, 0, sizeof(unsigned long long), st->stream));
+    st->vis_capacity = n;
+  }
+  if (!cb.hist) {
+    const Geometry geo = st->geometry();
+    st->nkeys = size_t(geo.ntx) * geo.nty * geo.nr * geo.nc;
+    HPGX_HIP(st, dev_alloc(cb.hist, (st->nkeys + 1) * sizeof(unsigned)));
+    HPGX_HIP(st, dev_alloc(cb.offsets, (st->nkeys + 1) * sizeof(unsigned)));
+    HPGX_HIP(st, dev_alloc(cb.cursor, (st->nkeys + 1) * sizeof(unsigned)));
+    cb.scan_temp_bytes = hpgx::scan_temp_bytes(st->nkeys);
+    HPGX_HIP(st, dev_alloc(cb.scan_temp, cb.scan_temp_bytes));
+  }
+  if (st->cf.ngroups > 0) {
+    const size_t need = st->vis_capacity * st->gsize[2]
+      * hpgx::tiles_per_footprint(st->cf.max_sx, st->cf.max_sy);
+    if (need > cb.entries_capacity) {
+      dev_free(cb.entries);
+      cb.entries_capacity = 0;
+      HPGX_HIP(st, dev_alloc(cb.entries, need * sizeof(unsigned)));
+      cb.entries_capacity = need;
+    }
+  }
+  return HPGX_SUCCESS;
+}
+
+int ensure_scratch(hpgx_state_t st) {
+  if (!st->scratch) HPGX_HIP(st, dev_alloc(st->scratch, st->grid_bytes()));
+  return HPGX_SUCCESS;
+}
+
+void free_state(hpgx_state_t st) {
+  if (st->stream) (void)hipStreamSynchronize(st->stream);
+  dev_free(st->grid);
+  dev_free(st->weights);
+  dev_free(st->model);
+  dev_free(st->scratch);
+  dev_free(st->cf.groups);
+  dev_free(st->cf.taps);
+  dev_free(st->cf.wsum);
+  dev_free(st->staging);
+  CallBuffers& cb = st->cb;
+  dev_free(cb.vis);
+  dev_free(cb.prep);
+  dev_free(cb.gvis);
+  dev_free(cb.hist);
+  dev_free(cb.offsets);
+  dev_free(cb.cursor);
+  dev_free(cb.entries);
+  dev_free(cb.scan_temp);
+  dev_free(cb.invalid_count);
+  if (st->plan_z2z.valid) hipfftDestroy(st->plan_z2z.h);
+  if (st->plan_c2c.valid) hipfftDestroy(st->plan_c2c.h);
+  for (auto& e : st->events)
+    if (e) (void)hipEventDestroy(e);
+  if (st->stream) (void)hipStreamDestroy(st->stream);
+}
+
+int set_device(hpgx_state_t st) {
+  HPGX_HIP(st, hipSetDevice(st->device));
+  return HPGX_SUCCESS;
+}
+
+int zero_grid(hpgx_state_t st) {
+  HPGX_HIP(st, hipMemsetAsync(st->grid, 0, st->grid_bytes(), st->stream));
+  HPGX_HIP(st, hipMemsetAsync(
+                 st->weights, 0, sizeof(double) * st->gsize[2] * st->gsize[3],
+                 st->stream));
+  return HPGX_SUCCESS;
+}
+
+// host-side transposes between Layout::Left (x fastest) and Layout::Right
+// (cube fastest) of a (x, y, mrow, cube) complex<double> array
+void left_to_right(const double2* src, double2* dst, const unsigned* g) {
+  const size_t nx = g[0], ny = g[1], nr = g[2], nc = g[3];
+  for (size_t c = 0; c < nc; ++c)
+    for (size_t r = 0; r < nr; ++r)
+      for (size_t y = 0; y < ny; ++y)
+        for (size_t x = 0; x < nx; ++x)
+          dst[((x * ny + y) * nr + r) * nc + c] = src[((c * nr + r) * ny + y) * nx + x];
+}
+void right_to_left(const double2* src, double2* dst, const unsigned* g) {
+  const size_t nx = g[0], ny = g[1], nr = g[2], nc = g[3];
+  for (size_t c = 0; c < nc; ++c)
+    for (size_t r = 0; r < nr; ++r)
+      for (size_t y = 0; y < ny; ++y)
+        for (size_t x = 0; x < nx; ++x)
+          dst[((c * nr + r) * ny + y) * nx + x] = src[((x * ny + y) * nr + r) * nc + c];
+}
+
+// copies a device Layout::Left grid-shaped array (double2 or float2) to host
+// complex<double> in `layout`
+int copy_grid_to_host(
+  hpgx_state_t st, const void* dev, bool is_float, void* dst, int layout) {
+  const size_t n = st->grid_elems();
+  const void* src = dev;
+  if (is_float) {
+    // widen to complex<double> on the device in a temporary buffer
+    double2* tmp = nullptr;
+    HPGX_HIP(st, dev_alloc(tmp, n * sizeof(double2)));
+    hipError_t e = hpgx::launch_widen(st->stream, static_cast<const float2*>(dev), tmp, n);
+    if (e == hipSuccess) {
+      if (layout == HPGX_LAYOUT_LEFT) {
+        e = hipMemcpyAsync(dst, tmp, n * sizeof(double2), hipMemcpyDeviceToHost, st->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+      } else {
+        std::vector<double2> h(n);
+        e = hipMemcpyAsync(h.data(), tmp, n * sizeof(double2), hipMemcpyDeviceToHost, st->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+        if (e == hipSuccess) left_to_right(h.data(), static_cast<double2*>(dst), st->gsize);
+      }
+    }
+    (void)hipStreamSynchronize(st->stream);
+    dev_free(tmp);
+    if (e != hipSuccess) return hip_fail(st, e, "grid readback");
+    return HPGX_SUCCESS;
+  }
+  if (layout == HPGX_LAYOUT_LEFT) {
+    HPGX_HIP(st, hipMemcpyAsync(dst, src, n * sizeof(double2), hipMemcpyDeviceToHost, st->stream));
+    HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  } else {
+    std::vector<double2> h(n);
+    HPGX_HIP(st, hipMemcpyAsync(h.data(), src, n * sizeof(double2), hipMemcpyDeviceToHost, st->stream));
+    HPGX_HIP(st, hipStreamSynchronize(st->stream));
+    left_to_right(h.data(), static_cast<double2*>(dst), st->gsize);
+  }
+  return HPGX_SUCCESS;
+}
+
+int get_plan(hpgx_state_t st, bool fp64, hipfftHandle& out) {
+  Plan& p = fp64 ? st->plan_z2z : st->plan_c2c;
+  if (!p.valid) {
+    int n[2] = {int(st->gsize[1]), int(st->gsize[0])};  // y slowest, x fastest
+    const int batch = int(st->gsize[2] * st->gsize[3]);
+    const int dist = n[0] * n[1];
+    hipfftResult r = hipfftPlanMany(
+      &p.h, 2, n, nullptr, 1, dist, nullptr, 1, dist,
+      fp64 ? HIPFFT_Z2Z : HIPFFT_C2C, batch);
+    if (r != HIPFFT_SUCCESS)
+      return fail(st, HPGX_ERR_OTHER, "hipfftPlanMany failed (" + std::to_string(int(r)) + ")");
+    p.valid = true;
+  }
+  if (hipfftSetStream(p.h, st->stream) != HIPFFT_SUCCESS)
+    return fail(st, HPGX_ERR_OTHER, "hipfftSetStream failed");
+  out = p.h;
+  return HPGX_SUCCESS;
+}
+
+// unnormalized 2-D FFT of every plane of a Layout::Left grid-shaped array
+int fft_planes(hpgx_state_t st, void* data, bool fp64, double norm, int sign,
+               bool in_place, void** result) {
+  hipfftHandle h;
+  if (int rc = get_plan(st, fp64, h)) return rc;
+  const int dir = sign == HPGX_FFT_NEGATIVE ? HIPFFT_FORWARD : HIPFFT_BACKWARD;
+  void* out = data;
+  if (!in_place) {
+    const size_t bytes = st->grid_elems() * (fp64 ? sizeof(double2) : sizeof(float2));
+    HPGX_HIP(st, dev_alloc(out, bytes));
+  }
+  hipfftResult r = fp64
+    ? hipfftExecZ2Z(h, static_cast<hipfftDoubleComplex*>(data),
+                    static_cast<hipfftDoubleComplex*>(out), dir)
+    : hipfftExecC2C(h, static_cast<hipfftComplex*>(data),
+                    static_cast<hipfftComplex*>(out), dir);
+  if (r != HIPFFT_SUCCESS) {
+    if (out != data) {
+      (void)hipStreamSynchronize(st->stream);
+      (void)hipFree(out);
+    }
+    return fail(st, HPGX_ERR_OTHER, "hipfftExec failed (" + std::to_string(int(r)) + ")");
+  }
+  if (out != data) {
+    // the reference allocates a new grid for an out-of-place FFT
+    // (src/hpg_runtime.hpp:1331-1347); the old one is released
+    HPGX_HIP(st, hipStreamSynchronize(st->stream));
+    (void)hipFree(data);
+  }
+  // GridNormalizer with norm (src/hpg_core.hpp:1863-1889)
+  if (norm != 1) {
+    HPGX_HIP(st, hpgx::launch_scale(
+                   st->stream, fp64 ? 0 : 1, out, 1.0 / norm, st->grid_elems()));
+  }
+  *result = out;
+  return HPGX_SUCCESS;
+}
+
+int upload_model(hpgx_state_t st, const void* values, int layout, int memory) {
+  const size_t n = st->grid_elems();
+  if (!st->model) HPGX_HIP(st, dev_alloc(st->model, n * sizeof(double2)));
+  if (layout == HPGX_LAYOUT_LEFT) {
+    HPGX_HIP(st, hipMemcpyAsync(
+                   st->model, values, n * sizeof(double2),
+                   memory == HPGX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                   st->stream));
+    HPGX_HIP(st, hipStreamSynchronize(st->stream));
+    return HPGX_SUCCESS;
+  }
+  std::vector<double2> right(n), left(n);
+  if (memory == HPGX_DEVICE) {
+    HPGX_HIP(st, hipMemcpy(right.data(), values, n * sizeof(double2), hipMemcpyDeviceToHost));
+  } else {
+    std::memcpy(right.data(), values, n * sizeof(double2));
+  }
+  right_to_left(right.data(), left.data(), st->gsize);
+  HPGX_HIP(st, hipMemcpyAsync(st->model, left.data(), n * sizeof(double2),
+                              hipMemcpyHostToDevice, st->stream));
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  return HPGX_SUCCESS;
+}
+
+// gridding / degridding pipeline on records already in cb.vis
+int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
+             bool do_grid, hpgx::GvisSource src, const float2* residuals) {
+  const Geometry geo = st->geometry();
+  CallBuffers& cb = st->cb;
+  const int np = int(st->npol);
+  bool gvis_ready = false;
+  if (do_degrid) {
+    HPGX_HIP(st, hpgx::launch_degrid(
+                   st->stream, np, n, geo, st->cf, st->mu, st->model, do_grid, cb));
+    gvis_ready = true;
+  }
+  if (!do_grid && !update_weights) return HPGX_SUCCESS;
+  if (do_grid)
+    HPGX_HIP(st, hipMemsetAsync(cb.hist, 0, (st->nkeys + 1) * sizeof(unsigned), st->stream));
+  HPGX_HIP(st, hpgx::launch_prep(
+                 st->stream, np, n, geo, st->cf, st->mu, update_weights, do_grid,
+                 gvis_ready, src, residuals, cb, st->weights));
+  if (!do_grid) return HPGX_SUCCESS;
+  HPGX_HIP(st, hpgx::launch_scan(st->stream, st->nkeys, cb));
+  HPGX_HIP(st, hpgx::launch_scatter(st->stream, np, n, geo, st->mu, cb, st->cf.groups));
+  HPGX_HIP(st, hpgx::launch_tile_grid(
+                 st->stream, np, st->mode == HPGX_GRID_FP64 ? 0 : 1, st->nkeys, geo,
+                 st->cf, st->mu, cb, st->grid));
+  return HPGX_SUCCESS;
+}
+
+int stage_records(hpgx_state_t st, const void* vis, size_t n, int memory) {
+  const size_t bytes = n * visdata_size(st->npol);
+  if (bytes == 0) return HPGX_SUCCESS;
+  HPGX_HIP(st, hipMemcpyAsync(
+                 st->cb.vis, vis, bytes,
+                 memory == HPGX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                 st->stream));
+  // a host source is not kept alive by the caller beyond this call
+  if (memory != HPGX_DEVICE) HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  return HPGX_SUCCESS;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int hpgx_initialize(int device_id) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    std::lock_guard<std::mutex> lk(g_create_mtx);
+    g_create_error = "no HIP device available";
+    return HPGX_ERR_DISABLED_DEVICE;
+  }
+  if (device_id >= count) return HPGX_ERR_DISABLED_DEVICE;
+  if (device_id >= 0) {
+    if (hipSetDevice(device_id) != hipSuccess) return HPGX_ERR_DISABLED_DEVICE;
+    g_device = device_id;
+  } else {
+    int d = 0;
+    (void)hipGetDevice(&d);
+    g_device = d;
+  }
+  return HPGX_SUCCESS;
+}
+
+void hpgx_finalize(void) {}
+
+int hpgx_device_count(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+  return count;
+}
+
+const char* hpgx_version(void) { return "hpgx 0.1.0 (gfx950)"; }
+
+int hpgx_create(
+  hpgx_state_t* out, unsigned max_added_tasks, size_t max_visibility_batch_size,
+  const unsigned grid_size[4], const double grid_scale[2],
+  unsigned num_polarizations, unsigned num_mueller_rows,
+  const int* mueller_indexes, const int* conjugate_mueller_indexes,
+  int grid_mode, unsigned init_cf_oversampling, unsigned init_cf_num_groups,
+  const unsigned* init_cf_extents) {
+  auto create_fail = [](int code, const std::string& msg) {
+    std::lock_guard<std::mutex> lk(g_create_mtx);
+    g_create_error = msg;
+    return code;
+  };
+  if (!out) return create_fail(HPGX_ERR_OTHER, "null state pointer");
+  *out = nullptr;
+  // GridderState::create validation order (src/hpg.cc:238-263)
+  if (num_mueller_rows != grid_size[2])
+    return create_fail(
+      HPGX_ERR_INVALID_NUMBER_MUELLER_INDEX_ROWS,
+      "Number of rows of Mueller indexes does not equal grid mrow extent");
+  if (hpgx_device_count() == 0)
+    return create_fail(HPGX_ERR_DISABLED_DEVICE, "Requested device is not enabled");
+  if (num_polarizations < 1 || num_polarizations > 4)
+    return create_fail(HPGX_ERR_INVALID_NUMBER_POLARIZATIONS,
+                       "number of polarizations must be 1..4");
+  if (num_mueller_rows > 4)
+    return create_fail(HPGX_ERR_INVALID_NUMBER_MUELLER_INDEX_ROWS,
+                       "at most 4 Mueller index rows are supported");
+  if (grid_mode != HPGX_GRID_FP64 && grid_mode != HPGX_GRID_FP32_COMPENSATED)
+    return create_fail(HPGX_ERR_OTHER, "unknown grid mode");
+  for (int d = 0; d < 4; ++d)
+    if (grid_size[d] == 0 || grid_size[d] > (1u << 30))
+      return create_fail(HPGX_ERR_OTHER, "grid extents must be in [1, 2^30]");
+
+  auto* st = new (std::nothrow) hpgx_state_s();
+  if (!st) return create_fail(HPGX_ERR_OTHER, "out of host memory");
+  int dev = g_device;
+  if (dev < 0) (void)hipGetDevice(&dev);
+  st->device = dev;
+  // max_added_tasks is limited to 0: one stream per state
+  (void)max_added_tasks;
+  st->max_added_tasks = 0;
+  st->max_batch = max_visibility_batch_size;
+  std::memcpy(st->gsize, grid_size, sizeof(st->gsize));
+  st->gscale[0] = grid_scale[0];
+  st->gscale[1] = grid_scale[1];
+  st->npol = num_polarizations;
+  st->mode = grid_mode;
+  // Mueller index matrices padded to 4x4 with -1
+  // (copy_mueller_indexes, src/hpg_runtime.hpp:1590-1614)
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      const bool in = unsigned(r) < num_mueller_rows && unsigned(c) < num_polarizations;
+      st->mu.M[r][c] = in ? mueller_indexes[r * num_polarizations + c] : -1;
+      st->mu.Mc[r][c] = in ? conjugate_mueller_indexes[r * num_polarizations + c] : -1;
+    }
+  st->mueller_rows.assign(mueller_indexes, mueller_indexes + num_mueller_rows * num_polarizations);
+  st->conj_mueller_rows.assign(
+    conjugate_mueller_indexes, conjugate_mueller_indexes + num_mueller_rows * num_polarizations);
+
+  int rc = HPGX_SUCCESS;
+  hipError_t e = hipSetDevice(st->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = dev_alloc(st->grid, st->grid_bytes());
+  if (e == hipSuccess)
+    e = dev_alloc(st->weights, sizeof(double) * grid_size[2] * grid_size[3]);
+  if (e != hipSuccess) rc = hip_fail(st, e, "state allocation");
+  if (!rc) rc = zero_grid(st);
+  if (!rc && init_cf_num_groups > 0 && init_cf_extents && init_cf_oversampling > 0)
+    rc = hpgx_allocate_convolution_function_region(
+      st, init_cf_oversampling, init_cf_num_groups, init_cf_extents);
+  if (!rc) rc = ensure_call_buffers(st);
+  if (!rc) {
+    e = hipStreamSynchronize(st->stream);
+    if (e != hipSuccess) rc = hip_fail(st, e, "state initialization");
+  }
+  if (rc) {
+    std::string msg = st->err;
+    free_state(st);
+    delete st;
+    return create_fail(rc, msg);
+  }
+  *out = st;
+  return HPGX_SUCCESS;
+}
+
+int hpgx_clone(hpgx_state_t src, hpgx_state_t* dst) {
+  if (!src || !dst) return HPGX_ERR_OTHER;
+  *dst = nullptr;
+  if (int rc = set_device(src)) return rc;
+  HPGX_HIP(src, hipStreamSynchronize(src->stream));
+  hpgx_state_t st = nullptr;
+  std::vector<unsigned> ext;
+  int rc = hpgx_create(
+    &st, src->max_added_tasks, src->max_batch, src->gsize, src->gscale, src->npol,
+    src->gsize[2], src->mueller_rows.data(), src->conj_mueller_rows.data(),
+    src->mode, 0, 0, nullptr);
+  if (rc) return fail(src, rc, "clone: " + std::string(g_create_error));
+  auto copy = [&](void* d, const void* s, size_t bytes) {
+    return hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, st->stream);
+  };
+  hipError_t e = copy(st->grid, src->grid, src->grid_bytes());
+  if (e == hipSuccess)
+    e = copy(st->weights, src->weights, sizeof(double) * src->gsize[2] * src->gsize[3]);
+  if (e == hipSuccess && src->model) {
+    e = dev_alloc(st->model, src->grid_elems() * sizeof(double2));
+    if (e == hipSuccess) e = copy(st->model, src->model, src->grid_elems() * sizeof(double2));
+  }
+  if (e == hipSuccess && src->cf.ngroups > 0) {
+    rc = ensure_cf_capacity(st, src->taps_capacity, src->wsum_capacity);
+    if (rc) {
+      const std::string msg = "clone: " + st->err;
+      hpgx_destroy(st);
+      return fail(src, rc, msg);
+    }
+    e = copy(st->cf.taps, src->cf.taps, src->taps_capacity * sizeof(float2));
+    if (e == hipSuccess)
+      e = copy(st->cf.wsum, src->cf.wsum, src->wsum_capacity * sizeof(double2));
+    if (e == hipSuccess)
+      e = copy(st->cf.groups, src->cf.groups, src->groups.size() * sizeof(GroupInfo));
+    GroupInfo* g = st->cf.groups;
+    float2* t = st->cf.taps;
+    double2* w = st->cf.wsum;
+    st->cf = src->cf;
+    st->cf.groups = g;
+    st->cf.taps = t;
+    st->cf.wsum = w;
+    st->groups = src->groups;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+  if (e != hipSuccess) {
+    hpgx_destroy(st);
+    return hip_fail(src, e, "clone");
+  }
+  if ((rc = ensure_call_buffers(st))) {
+    hpgx_destroy(st);
+    return fail(src, rc, "clone: call buffers");
+  }
+  *dst = st;
+  return HPGX_SUCCESS;
+}
+
+void hpgx_destroy(hpgx_state_t st) {
+  if (!st) return;
+  (void)hipSetDevice(st->device);
+  free_state(st);
+  delete st;
+}
+
+size_t hpgx_error_message(hpgx_state_t st, char* buf, size_t len) {
+  std::string msg;
+  if (st) {
+    msg = st->err;
+  } else {
+    std::lock_guard<std::mutex> lk(g_create_mtx);
+    msg = g_create_error;
+  }
+  if (buf && len > 0) {
+    const size_t k = std::min(len - 1, msg.size());
+    std::memcpy(buf, msg.data(), k);
+    buf[k] = '\0';
+  }
+  return msg.size();
+}
+
+unsigned hpgx_max_added_tasks(hpgx_state_t st) { return st ? st->max_added_tasks : 0; }
+size_t hpgx_max_visibility_batch_size(hpgx_state_t st) { return st ? st->max_batch : 0; }
+void hpgx_grid_size(hpgx_state_t st, unsigned g[4]) {
+  if (st) std::memcpy(g, st->gsize, sizeof(st->gsize));
+}
+void hpgx_grid_scale(hpgx_state_t st, double s[2]) {
+  if (st) {
+    s[0] = st->gscale[0];
+    s[1] = st->gscale[1];
+  }
+}
+unsigned hpgx_num_polarizations(hpgx_state_t st) { return st ? st->npol : 0; }
+int hpgx_grid_mode(hpgx_state_t st) { return st ? st->mode : -1; }
+
+size_t hpgx_convolution_function_region_size(
+  hpgx_state_t st, unsigned oversampling, unsigned num_groups,
+  const unsigned* extents) {
+  if (!extents || num_groups == 0 || oversampling == 0) return st ? st->taps_capacity : 0;
+  return cf_taps_for(oversampling, num_groups, extents);
+}
+
+int hpgx_allocate_convolution_function_region(
+  hpgx_state_t st, unsigned oversampling, unsigned num_groups,
+  const unsigned* extents) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (int rc = set_device(st)) return rc;
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  if (num_groups == 0 || !extents || oversampling == 0) {
+    dev_free(st->cf.taps);
+    dev_free(st->cf.wsum);
+    st->taps_capacity = st->wsum_capacity = 0;
+    st->cf.ngroups = 0;
+    st->groups.clear();
+    return HPGX_SUCCESS;
+  }
+  size_t ws = 0;
+  const size_t taps = cf_taps_for(oversampling, num_groups, extents, &ws);
+  // force reallocation to the requested size (src/hpg_runtime.hpp:871-880)
+  if (st->cf.ngroups == 0) {
+    dev_free(st->cf.taps);
+    dev_free(st->cf.wsum);
+    st->taps_capacity = st->wsum_capacity = 0;
+  }
+  return ensure_cf_capacity(st, taps, ws);
+}
+
+int hpgx_set_convolution_function(
+  hpgx_state_t st, unsigned oversampling, unsigned num_groups,
+  const unsigned* extents, const void* const* values, int memory) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (oversampling == 0)
+    return fail(st, HPGX_ERR_OTHER, "CF oversampling must be positive");
+  if (num_groups > unsigned(hpgx::max_cf_groups))
+    return fail(st, HPGX_ERR_OTHER, "too many CF groups");
+  // CFSupportExceedsGrid check (src/hpg_runtime.hpp:892-902)
+  for (unsigned g = 0; g < num_groups; ++g) {
+    const unsigned* e = extents + 4 * g;
+    if (size_t(e[0]) > size_t(st->gsize[0]) * oversampling
+        || size_t(e[1]) > size_t(st->gsize[1]) * oversampling)
+      return fail(st, HPGX_ERR_CF_SUPPORT_EXCEEDS_GRID, "CF support size exceeds grid size");
+    if (e[0] < 2 * hpgx::cf_padding * oversampling || e[1] < 2 * hpgx::cf_padding * oversampling)
+      return fail(st, HPGX_ERR_OTHER, "CF extents smaller than the padding");
+  }
+  if (int rc = set_device(st)) return rc;
+  // the previous CF may still be in use by queued kernels
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+
+  const int os = int(oversampling);
+  const int F = (os + 1) / 2;
+  const int nf = 2 * F + 1;
+  std::vector<GroupInfo> groups(num_groups);
+  long long toff = 0, woff = 0;
+  int max_sx = 1, max_sy = 1;
+  size_t max_stage = 0;
+  for (unsigned g = 0; g < num_groups; ++g) {
+    const unsigned* e = extents + 4 * g;
+    GroupInfo& gi = groups[g];
+    radii(e, oversampling, gi.rx, gi.ry);
+    gi.sx = 2 * gi.rx + 1;
+    gi.sy = 2 * gi.ry + 1;
+    gi.nmueller = int(e[2]);
+    gi.ncube = int(e[3]);
+    gi.tap_offset = toff;
+    gi.wsum_offset = woff;
+    const long long slices = (long long)gi.ncube * gi.nmueller * nf * nf;
+    toff += slices * gi.sx * gi.sy;
+    woff += slices;
+    max_sx = std::max(max_sx, gi.sx);
+    max_sy = std::max(max_sy, gi.sy);
+    max_stage = std::max(max_stage, size_t(e[0]) * e[1] * e[2] * e[3]);
+  }
+  if (int rc = ensure_cf_capacity(st, size_t(toff), size_t(woff))) return rc;
+  if (memory != HPGX_DEVICE && max_stage > st->staging_capacity) {
+    dev_free(st->staging);
+    st->staging_capacity = 0;
+    HPGX_HIP(st, dev_alloc(st->staging, max_stage * sizeof(float2)));
+    st->staging_capacity = max_stage;
+  }
+  for (unsigned g = 0; g < num_groups; ++g) {
+    const unsigned* e = extents + 4 * g;
+    const size_t count = size_t(e[0]) * e[1] * e[2] * e[3];
+    const float2* src = static_cast<const float2*>(values[g]);
+    if (memory != HPGX_DEVICE) {
+      HPGX_HIP(st, hipMemcpyAsync(st->staging, values[g], count * sizeof(float2),
+                                  hipMemcpyHostToDevice, st->stream));
+      src = st->staging;
+    }
+    HPGX_HIP(st, hpgx::launch_cf_relayout(st->stream, src, e, groups[g], os, F, nf, st->cf.taps));
+    HPGX_HIP(st, hpgx::launch_cf_wsum(st->stream, groups[g], nf, st->cf.taps, st->cf.wsum));
+    if (memory != HPGX_DEVICE) HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  }
+  if (num_groups > 0)
+    HPGX_HIP(st, hipMemcpyAsync(st->cf.groups, groups.data(), num_groups * sizeof(GroupInfo),
+                                hipMemcpyHostToDevice, st->stream));
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  st->groups = groups;
+  st->cf.os = os;
+  st->cf.F = F;
+  st->cf.nf = nf;
+  st->cf.ngroups = int(num_groups);
+  st->cf.max_sx = max_sx;
+  st->cf.max_sy = max_sy;
+  return ensure_call_buffers(st);
+}
+
+int hpgx_set_model(
+  hpgx_state_t st, const unsigned model_size[4], const void* values, int layout,
+  int memory) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (std::memcmp(model_size, st->gsize, sizeof(st->gsize)) != 0) {
+    char buf[256];
+    std::snprintf(buf, sizeof(buf),
+                  "model grid size [%u,%u,%u,%u] is different from visibility grid size [%u,%u,%u,%u]",
+                  model_size[0], model_size[1], model_size[2], model_size[3],
+                  st->gsize[0], st->gsize[1], st->gsize[2], st->gsize[3]);
+    return fail(st, HPGX_ERR_INVALID_MODEL_GRID_SIZE, buf);
+  }
+  if (int rc = set_device(st)) return rc;
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  return upload_model(st, values, layout, memory);
+}
+
+int hpgx_reset_model(hpgx_state_t st) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (int rc = set_device(st)) return rc;
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  dev_free(st->model);
+  return HPGX_SUCCESS;
+}
+
+int hpgx_has_model(hpgx_state_t st) { return st && st->model ? 1 : 0; }
+
+int hpgx_grid_visibilities(
+  hpgx_state_t st, const void* vis, size_t n, unsigned npol,
+  int update_grid_weights, int do_degrid, int return_visibilities, int do_grid,
+  void* vis_out, int memory) {
+  if (!st) return HPGX_ERR_OTHER;
+  // validation order of runtime::GridderState::grid_visibilities
+  // (src/hpg_runtime.hpp:1815-1827)
+  if (n > st->max_batch)
+    return fail(st, HPGX_ERR_EXCESSIVE_NUMBER_VISIBILITIES,
+                "Number of visibilities exceeds maximum batch size");
+  if (npol != st->npol)
+    return fail(st, HPGX_ERR_INVALID_NUMBER_POLARIZATIONS,
+                "Number of visibility polarizations does not match Mueller matrix");
+  if (!do_grid && update_grid_weights)
+    return fail(st, HPGX_ERR_UPDATE_WEIGHTS_WITHOUT_GRIDDING,
+                "Grid weights cannot be updated without doing gridding");
+  if (n == 0) return HPGX_SUCCESS;
+  if (!vis) return fail(st, HPGX_ERR_OTHER, "null visibilities");
+  if (return_visibilities && !vis_out)
+    return fail(st, HPGX_ERR_OTHER, "null visibilities_out");
+  if (int rc = set_device(st)) return rc;
+  if (st->cf.ngroups == 0) {
+    // no CF: every record refers to a missing CF group and is skipped
+    if (return_visibilities && vis_out != vis) {
+      HPGX_HIP(st, hipMemcpyAsync(
+                     vis_out, vis, n * visdata_size(npol),
+                     memory == HPGX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToHost,
+                     st->stream));
+    }
+    return HPGX_SUCCESS;
+  }
+  if (int rc = ensure_call_buffers(st)) return rc;
+  if (int rc = stage_records(st, vis, n, memory)) return rc;
+  if (int rc = run_grid(st, n, update_grid_weights, do_degrid, do_grid,
+                        hpgx::GvisSource::Values, nullptr))
+    return rc;
+  if (return_visibilities) {
+    HPGX_HIP(st, hipMemcpyAsync(
+                   vis_out, st->cb.vis, n * visdata_size(npol),
+                   memory == HPGX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                   st->stream));
+  }
+  return HPGX_SUCCESS;
+}
+
+int hpgx_degrid_adjoint(
+  hpgx_state_t st, const void* vis, const void* residuals, size_t n,
+  unsigned npol, int memory) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (n > st->max_batch)
+    return fail(st, HPGX_ERR_EXCESSIVE_NUMBER_VISIBILITIES,
+                "Number of visibilities exceeds maximum batch size");
+  if (npol != st->npol)
+    return fail(st, HPGX_ERR_INVALID_NUMBER_POLARIZATIONS,
+                "Number of visibility polarizations does not match Mueller matrix");
+  if (n == 0 || st->cf.ngroups == 0) return HPGX_SUCCESS;
+  if (!vis || !residuals) return fail(st, HPGX_ERR_OTHER, "null argument");
+  if (int rc = set_device(st)) return rc;
+  if (int rc = ensure_call_buffers(st)) return rc;
+  if (int rc = stage_records(st, vis, n, memory)) return rc;
+  // residuals staged into the gvis buffer's tail is not possible (gvis is the
+  // output); use a temporary device copy when they come from the host
+  const float2* res = static_cast<const float2*>(residuals);
+  float2* tmp = nullptr;
+  if (memory != HPGX_DEVICE) {
+    HPGX_HIP(st, dev_alloc(tmp, n * npol * sizeof(float2)));
+    HPGX_HIP(st, hipMemcpyAsync(tmp, residuals, n * npol * sizeof(float2),
+                                hipMemcpyHostToDevice, st->stream));
+    res = tmp;
+  }
+  int rc = run_grid(st, n, false, false, true, hpgx::GvisSource::Adjoint, res);
+  if (tmp) {
+    (void)hipStreamSynchronize(st->stream);
+    dev_free(tmp);
+  }
+  return rc;
+}
+
+int hpgx_fence(hpgx_state_t st) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (int rc = set_device(st)) return rc;
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  unsigned long long invalid = 0;
+  if (st->cb.invalid_count) {
+    HPGX_HIP(st, hipMemcpy(&invalid, st->cb.invalid_count, sizeof(invalid), hipMemcpyDeviceToHost));
+    if (invalid) {
+      HPGX_HIP(st, hipMemset(st->cb.invalid_count, 0, sizeof(invalid)));
+      return fail(st, HPGX_ERR_OUT_OF_BOUNDS_CF_INDEX,
+                  std::to_string(invalid)
+                    + " visibilities skipped: cf_index or grid_cube out of bounds");
+    }
+  }
+  return HPGX_SUCCESS;
+}
+
+int hpgx_normalize_by_weights(hpgx_state_t st, double wfactor) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (int rc = set_device(st)) return rc;
+  HPGX_HIP(st, hpgx::launch_normalize(
+                 st->stream, st->mode == HPGX_GRID_FP64 ? 0 : 1, st->grid, st->weights,
+                 wfactor, st->geometry()));
+  return HPGX_SUCCESS;
+}
+
+int hpgx_apply_grid_fft(hpgx_state_t st, double norm, int sign, int in_place) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (int rc = set_device(st)) return rc;
+  void* out = nullptr;
+  if (int rc = fft_planes(st, st->grid, st->mode == HPGX_GRID_FP64, norm, sign,
+                          in_place != 0, &out))
+    return rc;
+  st->grid = out;
+  return HPGX_SUCCESS;
+}
+
+int hpgx_apply_model_fft(hpgx_state_t st, double norm, int sign, int in_place) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (!st->model) return HPGX_SUCCESS;
+  if (int rc = set_device(st)) return rc;
+  void* out = nullptr;
+  if (int rc = fft_planes(st, st->model, true, norm, sign, in_place != 0, &out)) return rc;
+  st->model = static_cast<double2*>(out);
+  return HPGX_SUCCESS;
+}
+
+int hpgx_shift_grid(hpgx_state_t st, int direction) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (int rc = set_device(st)) return rc;
+  if (int rc = ensure_scratch(st)) return rc;
+  HPGX_HIP(st, hpgx::launch_shift(
+                 st->stream, st->mode == HPGX_GRID_FP64 ? 0 : 1, st->grid, st->scratch,
+                 direction, st->geometry()));
+  return HPGX_SUCCESS;
+}
+
+int hpgx_shift_model(hpgx_state_t st, int direction) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (!st->model) return HPGX_SUCCESS;
+  if (int rc = set_device(st)) return rc;
+  const Geometry geo = st->geometry();
+  void* scratch = nullptr;
+  const bool even = geo.nx % 2 == 0 && geo.ny % 2 == 0;
+  if (!even) HPGX_HIP(st, dev_alloc(scratch, st->grid_elems() * sizeof(double2)));
+  hipError_t e = hpgx::launch_shift(st->stream, 0, st->model, scratch, direction, geo);
+  if (scratch) {
+    (void)hipStreamSynchronize(st->stream);
+    dev_free(scratch);
+  }
+  if (e != hipSuccess) return hip_fail(st, e, "shift_model");
+  return HPGX_SUCCESS;
+}
+
+int hpgx_reset_grid(hpgx_state_t st) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (int rc = set_device(st)) return rc;
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  if (int rc = zero_grid(st)) return rc;
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  return HPGX_SUCCESS;
+}
+
+int hpgx_grid_values(hpgx_state_t st, void* dst, int layout) {
+  if (!st || !dst) return HPGX_ERR_OTHER;
+  if (int rc = set_device(st)) return rc;
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  return copy_grid_to_host(st, st->grid, st->mode != HPGX_GRID_FP64, dst, layout);
+}
+
+int hpgx_grid_weights(hpgx_state_t st, double* dst, int layout) {
+  if (!st || !dst) return HPGX_ERR_OTHER;
+  if (int rc = set_device(st)) return rc;
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  const size_t nr = st->gsize[2], nc = st->gsize[3];
+  std::vector<double> h(nr * nc);
+  HPGX_HIP(st, hipMemcpy(h.data(), st->weights, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (layout == HPGX_LAYOUT_LEFT) {
+    std::memcpy(dst, h.data(), h.size() * sizeof(double));
+  } else {
+    for (size_t c = 0; c < nc; ++c)
+      for (size_t r = 0; r < nr; ++r) dst[r * nc + c] = h[c * nr + r];
+  }
+  return HPGX_SUCCESS;
+}
+
+int hpgx_model_values(hpgx_state_t st, void* dst, int layout) {
+  if (!st || !dst) return HPGX_ERR_OTHER;
+  if (int rc = set_device(st)) return rc;
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  if (!st->model) {
+    // an unset model reads as zero (UnallocatedModelValueArray,
+    // src/hpg_runtime.hpp:1230-1240)
+    std::memset(dst, 0, st->grid_elems() * sizeof(double2));
+    return HPGX_SUCCESS;
+  }
+  return copy_grid_to_host(st, st->model, false, dst, layout);
+}
+
+int hpgx_grid_window(
+  hpgx_state_t st, unsigned x0, unsigned y0, unsigned nx, unsigned ny,
+  unsigned mrow, unsigned cube, void* dst) {
+  if (!st || !dst) return HPGX_ERR_OTHER;
+  if (size_t(x0) + nx > st->gsize[0] || size_t(y0) + ny > st->gsize[1]
+      || mrow >= st->gsize[2] || cube >= st->gsize[3])
+    return fail(st, HPGX_ERR_OTHER, "grid window out of bounds");
+  if (int rc = set_device(st)) return rc;
+  HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  const size_t plane = size_t(cube) * st->gsize[2] + mrow;
+  const size_t ps = size_t(st->gsize[0]) * st->gsize[1];
+  std::vector<float2> row32;
+  for (unsigned y = 0; y < ny; ++y) {
+    const size_t off = plane * ps + size_t(y0 + y) * st->gsize[0] + x0;
+    double2* d = static_cast<double2*>(dst) + size_t(y) * nx;
+    if (st->mode == HPGX_GRID_FP64) {
+      HPGX_HIP(st, hipMemcpy(d, static_cast<double2*>(st->grid) + off,
+                             nx * sizeof(double2), hipMemcpyDeviceToHost));
+    } else {
+      row32.resize(nx);
+      HPGX_HIP(st, hipMemcpy(row32.data(), static_cast<float2*>(st->grid) + off,
+                             nx * sizeof(float2), hipMemcpyDeviceToHost));
+      for (unsigned x = 0; x < nx; ++x)
+        d[x] = make_double2(double(row32[x].x), double(row32[x].y));
+    }
+  }
+  return HPGX_SUCCESS;
+}
+
+int hpgx_grid_device_ptr(hpgx_state_t st, void** ptr, size_t* count) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (ptr) *ptr = st->grid;
+  if (count) *count = st->grid_elems();
+  return HPGX_SUCCESS;
+}
+
+int hpgx_event_record(hpgx_state_t st, int slot) {
+  if (!st || slot < 0 || slot >= num_event_slots) return HPGX_ERR_OTHER;
+  if (int rc = set_device(st)) return rc;
+  if (!st->events[slot]) HPGX_HIP(st, hipEventCreate(&st->events[slot]));
+  HPGX_HIP(st, hipEventRecord(st->events[slot], st->stream));
+  return HPGX_SUCCESS;
+}
+
+int hpgx_event_elapsed(hpgx_state_t st, int slot0, int slot1, float* ms) {
+  if (!st || !ms || slot0 < 0 || slot1 < 0 || slot0 >= num_event_slots
+      || slot1 >= num_event_slots || !st->events[slot0] || !st->events[slot1])
+    return HPGX_ERR_OTHER;
+  HPGX_HIP(st, hipEventSynchronize(st->events[slot1]));
+  HPGX_HIP(st, hipEventElapsedTime(ms, st->events[slot0], st->events[slot1]));
+  return HPGX_SUCCESS;
+}
+
+}  // extern "C"

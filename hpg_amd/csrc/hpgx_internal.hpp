@@ -1,0 +1,134 @@
+// Internal declarations shared by the C ABI (hpgx_api.cpp) and the HIP
+// kernels (hpgx_kernels.hip).  Not installed; the public interface is
+// include/hpgx.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace hpgx {
+
+// CFArray::padding (src/hpg.hpp:655)
+constexpr int cf_padding = 2;
+// maximum number of CF groups (HPG_MAX_NUM_CF_GROUPS, src/CMakeLists.txt:37)
+constexpr int max_cf_groups = 1000;
+// grid tile edge (pixels) owned by one workgroup of the gridding kernel
+constexpr int tile = 32;
+// waves per gridding workgroup; each wave accumulates into its own tile copy
+constexpr int tile_waves = 4;
+
+// Per-group geometry of the device CF.  The device layout of group g is
+// "fine-offset sliced": for every (cube, mueller, fx, fy) -- fx, fy in
+// [-F, F] the visibility's fine offset on each axis, F = (os + 1) / 2 -- the
+// sx x sy taps of the support are contiguous, x fastest:
+//   taps[tap_offset + (((cube * nmueller + m) * nf + fx + F) * nf + fy + F)
+//        * sx * sy + Y * sx + X]
+//   = CFArray(ix, iy, m, cube, g), ix = (X + padding) * os + fx, iy likewise,
+// which is the element the reference reaches through its 6-D layout
+// (src/hpg_impl.hpp:221-281, src/hpg_core.hpp:422-446: major * os + minor).
+// wsum[wsum_offset + ((cube * nmueller + m) * nf + fx + F) * nf + fy + F]
+// is the FP64 sum of that slice's taps, X outer and Y inner.
+struct GroupInfo {
+  long long tap_offset;
+  long long wsum_offset;
+  int sx, sy;          // support 2r + 1 per axis
+  int rx, ry;          // radii (src/hpg.hpp:687-694)
+  int nmueller, ncube;
+};
+
+struct Mueller {
+  int M[4][4];   // mueller_indexes, padded with -1
+  int Mc[4][4];  // conjugate_mueller_indexes, padded with -1
+};
+
+// grid / CF geometry passed by value to kernels
+struct Geometry {
+  int nx, ny, nr, nc;  // grid_size: x, y, mrow, cube
+  double scale[2];     // grid_scale
+  int os;              // CF oversampling
+  int F;               // max |fine offset| = (os + 1) / 2
+  int nf;              // 2F + 1
+  int ngroups;
+  int ntx, nty;        // tiles per plane along x and y
+};
+
+// Per-visibility values computed once by the preparation kernel.
+struct PrepVis {
+  int gx, gy;            // grid coordinate of the support's corner
+  short fxi, fyi;        // fine offset + F
+  unsigned short grp;    // CF group
+  unsigned short flags;  // bit 0: pos_w; bit 1: inside grid; bit 2: gradient
+  unsigned cfcube;       // CF cube
+  unsigned gcube;        // grid cube
+};
+constexpr unsigned short flag_pos_w = 1;
+constexpr unsigned short flag_inside = 2;
+constexpr unsigned short flag_gradient = 4;
+
+enum class GvisSource { Values, Adjoint };
+
+// Device buffers for one gridding call (owned by the state).
+struct CallBuffers {
+  void* vis;            // raw VisData<N> records
+  PrepVis* prep;        // per visibility
+  float2* gvis;         // per visibility x N
+  unsigned* hist;       // per (plane, tile) key, nkeys + 1
+  unsigned* offsets;    // exclusive scan of hist, nkeys + 1
+  unsigned* cursor;     // scatter positions, nkeys
+  unsigned* entries;    // (vis << 2) | gpol, sorted by key
+  size_t entries_capacity;
+  void* scan_temp;
+  size_t scan_temp_bytes;
+  unsigned long long* invalid_count;
+};
+
+struct DeviceCF {
+  GroupInfo* groups;  // device array
+  float2* taps;
+  double2* wsum;
+  int os, F, nf, ngroups;
+  int max_sx, max_sy;
+};
+
+// ---- launchers (hpgx_kernels.hip) -------------------------------------
+
+hipError_t launch_cf_relayout(
+  hipStream_t s, const float2* staging, const unsigned ext[4],
+  const GroupInfo& gi, int os, int F, int nf, float2* taps);
+hipError_t launch_cf_wsum(
+  hipStream_t s, const GroupInfo& gi, int nf, const float2* taps,
+  double2* wsum);
+
+size_t tiles_per_footprint(int max_sx, int max_sy);
+
+// prepares visibilities and counts (plane, tile) entries
+hipError_t launch_prep(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, bool update_weights, bool do_grid, bool gvis_ready,
+  GvisSource src, const float2* residuals, CallBuffers& cb, double* weights);
+hipError_t launch_scan(hipStream_t s, size_t nkeys, CallBuffers& cb);
+size_t scan_temp_bytes(size_t nkeys);
+hipError_t launch_scatter(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const Mueller& mu,
+  CallBuffers& cb, const GroupInfo* groups);
+hipError_t launch_tile_grid(
+  hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
+  const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid);
+hipError_t launch_degrid(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, const double2* model, bool do_grid, CallBuffers& cb);
+
+hipError_t launch_normalize(
+  hipStream_t s, int mode, void* grid, const double* weights, double wfactor,
+  const Geometry& geo);
+hipError_t launch_scale(
+  hipStream_t s, int mode, void* grid, double inv_norm, size_t count);
+hipError_t launch_shift(
+  hipStream_t s, int mode, void* grid, void* scratch, int direction,
+  const Geometry& geo);
+hipError_t launch_widen(hipStream_t s, const float2* src, double2* dst, size_t count);
+hipError_t launch_narrow(hipStream_t s, const double2* src, float2* dst, size_t count);
+
+}  // namespace hpgx

@@ -1,0 +1,914 @@
+// HIP kernels of the MI355X gridding device (gfx950, wave64).
+//
+// Build with -ffp-contract=off: every FP32 product and FP64 sum below is
+// rounded where the reference's source rounds it (no fused multiply-add), so
+// that the FP64-grid mode reproduces the reference's per-tap values bit for
+// bit and differs from it only by the order of FP64 accumulation.
+//
+// Gridding pipeline for one batch of visibilities (VisibilityGridder::kernel,
+// src/hpg_core.hpp:1082-1253, re-organized for the device):
+//   k_prep     one thread per visibility: coordinates, skip rule, pre-phased
+//              values (gvis), FP64 weight sums, and a count of the
+//              (plane, tile) bins its support touches;
+//   scan       exclusive sum of the bin counts (rocPRIM via hipCUB);
+//   k_scatter  one thread per visibility: writes (visibility, gpol) entries
+//              into their bins;
+//   k_tile     one workgroup per bin: each wave accumulates its share of the
+//              bin's entries into a private LDS copy of the 32x32 tile (FP64,
+//              or FP32 with two-sum compensation), then the workgroup folds
+//              the copies and adds the tile into the grid with one coalesced
+//              read-modify-write.  Tiles partition each plane, so no global
+//              atomics touch the grid.
+// Degridding (degrid_vis, src/hpg_core.hpp:579-682) is one wave per
+// visibility: lanes gather model pixels and CF taps across the support and
+// the wave reduces with cross-lane shuffles.
+
+#include "hpgx_internal.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+namespace hpgx {
+namespace {
+
+constexpr double c_light = 299792458.0;  // src/hpg_core.hpp:527
+constexpr int wave = 64;
+constexpr int block = 256;
+
+// hpg::VisData<N> as seen by the device (src/hpg.hpp:236-311)
+template <int N>
+struct VisRec {
+  float2 vis[N];
+  float w[N];
+  double freq;
+  double phase;
+  double uvw[3];
+  unsigned grid_cube;
+  unsigned cf_index[2];  // {cube, grp}
+  double grad[2];
+};
+static_assert(sizeof(VisRec<1>) == 88, "VisData<1>");
+static_assert(sizeof(VisRec<2>) == 96, "VisData<2>");
+static_assert(sizeof(VisRec<3>) == 112, "VisData<3>");
+static_assert(sizeof(VisRec<4>) == 120, "VisData<4>");
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// complex<float> * complex<double> -> complex<double> (Kokkos mixed product)
+__device__ __forceinline__ double2 cmul(float2 a, double2 b) {
+  return make_double2(
+    double(a.x) * b.x - double(a.y) * b.y, double(a.x) * b.y + double(a.y) * b.x);
+}
+// Kokkos 4.x complex division: scale by the divisor's 1-norm
+__device__ __forceinline__ double2 kdiv(double2 x, double2 y) {
+  const double s = fabs(y.x) + fabs(y.y);
+  if (s == 0.0) return make_double2(x.x / s, x.y / s);
+  const double2 xs = make_double2(x.x / s, x.y / s);
+  const double2 ycs = make_double2(y.x / s, -y.y / s);
+  const double yabs = ycs.x * ycs.x + ycs.y * ycs.y;
+  double2 r = cmul(xs, ycs);
+  r.x /= yabs;
+  r.y /= yabs;
+  return r;
+}
+__device__ __forceinline__ double2 cphase(double ph) {
+  double s, c;
+  sincos(ph, &s, &c);
+  return make_double2(c, s);
+}
+__device__ __forceinline__ float2 narrow(double2 a) {
+  return make_float2(float(a.x), float(a.y));
+}
+// gvis = fl32(vis * phasor * weight) (src/hpg_core.hpp:1172-1182)
+__device__ __forceinline__ float2 prephase(float2 v, double2 phasor, float w) {
+  const double2 t = cmul(v, phasor);
+  return make_float2(float(t.x * double(w)), float(t.y * double(w)));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = wave / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, wave);
+  return v;
+}
+
+// q / d for 0 <= q < 2^32 / d, with m = magic(d) (exact in that range)
+__device__ __forceinline__ unsigned magic(unsigned d) {
+  return d == 1u ? 0u : 0xFFFFFFFFu / d + 1u;
+}
+__device__ __forceinline__ unsigned div_magic(unsigned q, unsigned d, unsigned m) {
+  return d == 1u ? q : __umulhi(q, m);
+}
+
+__device__ __forceinline__ int uniform(int v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// compute_vis_coord + Vis<N> ctor + all_within_grid
+// (src/hpg_core.hpp:422-446, 493-550, 1070-1080)
+struct Coord {
+  int gx, gy;    // grid coordinate of the support corner
+  int fx, fy;    // fine offsets
+  bool inside;
+};
+
+__device__ __forceinline__ bool axis_coord(
+  int g_size, int os, int radius, int support, double coord, double inv_lambda,
+  double scale, int& corner, int& fine) {
+  corner = 0;
+  fine = 0;
+  const double position = scale * coord * inv_lambda + g_size / 2.0;
+  if (!(fabs(position) < 1.0e9)) return false;
+  const double G = rint(position);
+  fine = int(rint((G - position) * os));
+  corner = int(G) - radius;
+  return 0 <= corner && corner + support <= g_size;
+}
+
+template <int N>
+__device__ __forceinline__ Coord vis_coord(
+  const VisRec<N>& r, const Geometry& geo, const GroupInfo& g) {
+  Coord c;
+  const double inv_lambda = r.freq / c_light;
+  const bool ix = axis_coord(
+    geo.nx, geo.os, g.rx, g.sx, r.uvw[0], inv_lambda, geo.scale[0], c.gx, c.fx);
+  const bool iy = axis_coord(
+    geo.ny, geo.os, g.ry, g.sy, r.uvw[1], inv_lambda, geo.scale[1], c.gy, c.fy);
+  c.inside = ix && iy;
+  return c;
+}
+
+__device__ __forceinline__ int tiles_begin(int corner) { return corner / tile; }
+__device__ __forceinline__ int tiles_end(int corner, int support) {
+  return (corner + support - 1) / tile + 1;
+}
+
+__device__ __forceinline__ bool row_has_index(const int* row, int npol) {
+  bool any = false;
+  for (int v = 0; v < npol; ++v) any |= row[v] >= 0;
+  return any;
+}
+
+// W_v = sum over grid rows of the degridding CF slice sums
+// (degrid_vis weights, src/hpg_core.hpp:643-668): with the im factor applied
+// to every tap, the sum is T[m] (w > 0) or conj(T[m]) (w <= 0).
+template <int N>
+__device__ __forceinline__ void degrid_norms(
+  const Geometry& geo, const GroupInfo& g, const double2* wsum,
+  const Mueller& mu, bool pos_w, unsigned cfcube, int fxi, int fyi,
+  double2* W) {
+  for (int v = 0; v < N; ++v) W[v] = make_double2(0.0, 0.0);
+  for (int gpol = 0; gpol < geo.nr; ++gpol) {
+    const int* row = pos_w ? mu.Mc[gpol] : mu.M[gpol];
+    for (int v = 0; v < N; ++v) {
+      const int m = row[v];
+      if (m >= 0 && m < g.nmueller) {
+        const double2 t = wsum[g.wsum_offset
+                               + ((long long)(cfcube * g.nmueller + m) * geo.nf + fxi)
+                                 * geo.nf + fyi];
+        W[v].x += t.x;
+        W[v].y += pos_w ? t.y : -t.y;
+      }
+    }
+  }
+}
+
+// ---- CF ingestion ----------------------------------------------------
+
+__global__ void k_cf_relayout(
+  const float2* __restrict__ staging, unsigned ex, unsigned ey, GroupInfo gi,
+  int os, int F, int nf, float2* __restrict__ taps, long long total) {
+  const long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (o >= total) return;
+  long long q = o;
+  const int X = int(q % gi.sx);
+  q /= gi.sx;
+  const int Y = int(q % gi.sy);
+  q /= gi.sy;
+  const int fyi = int(q % nf);
+  q /= nf;
+  const int fxi = int(q % nf);
+  q /= nf;
+  const int m = int(q % gi.nmueller);
+  const int c = int(q / gi.nmueller);
+  const long long ix = (long long)(X + cf_padding) * os + (fxi - F);
+  const long long iy = (long long)(Y + cf_padding) * os + (fyi - F);
+  float2 v = make_float2(0.0f, 0.0f);
+  if (ix >= 0 && iy >= 0 && ix < ex && iy < ey)
+    v = staging[((long long)(c * gi.nmueller + m) * ey + iy) * ex + ix];
+  taps[gi.tap_offset + o] = v;
+}
+
+__global__ void k_cf_wsum(
+  GroupInfo gi, int nf, const float2* __restrict__ taps,
+  double2* __restrict__ wsum, long long nslices) {
+  const long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (s >= nslices) return;
+  const float2* t = taps + gi.tap_offset + s * gi.sx * gi.sy;
+  double re = 0.0, im = 0.0;
+  for (int X = 0; X < gi.sx; ++X)
+    for (int Y = 0; Y < gi.sy; ++Y) {
+      const float2 c = t[Y * gi.sx + X];
+      re += double(c.x);
+      im += double(c.y);
+    }
+  wsum[gi.wsum_offset + s] = make_double2(re, im);
+}
+
+// ---- gridding ----------------------------------------------------------
+
+template <int N>
+__global__ __launch_bounds__(block) void k_prep(
+  const VisRec<N>* __restrict__ recs, size_t n, Geometry geo,
+  const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
+  Mueller mu, int update_weights, int do_grid, int gvis_ready, int adjoint,
+  const float2* __restrict__ residuals, PrepVis* __restrict__ prep,
+  float2* __restrict__ gvis, unsigned* __restrict__ hist,
+  double* __restrict__ weights, unsigned long long* __restrict__ invalid) {
+
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  bool counted = false;  // contributes to the weights
+  unsigned gcube = 0;
+  double tw[4] = {0.0, 0.0, 0.0, 0.0};
+
+  if (i < n) {
+    const VisRec<N>& r = recs[i];
+    const unsigned grp = r.cf_index[1];
+    const unsigned cfcube = r.cf_index[0];
+    gcube = r.grid_cube;
+    PrepVis p;
+    p.gx = p.gy = 0;
+    p.fxi = p.fyi = 0;
+    p.grp = 0;
+    p.flags = 0;
+    p.cfcube = cfcube;
+    p.gcube = gcube;
+    bool valid = grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc);
+    if (valid) valid = cfcube < unsigned(groups[grp].ncube);
+    if (!valid) {
+      atomicAdd(invalid, 1ull);
+    } else {
+      const GroupInfo g = groups[grp];
+      const Coord c = vis_coord(r, geo, g);
+      const bool pos_w = r.uvw[2] > 0;
+      p.gx = c.gx;
+      p.gy = c.gy;
+      p.fxi = short(c.fx + geo.F);
+      p.fyi = short(c.fy + geo.F);
+      p.grp = (unsigned short)grp;
+      p.flags = (pos_w ? flag_pos_w : 0) | (c.inside ? flag_inside : 0)
+        | ((r.grad[0] != 0.0 || r.grad[1] != 0.0) ? flag_gradient : 0);
+      const double2 phasor = cphase(r.phase);
+      if (!gvis_ready) {
+        if (!adjoint) {
+          for (int v = 0; v < N; ++v)
+            gvis[i * N + v] = prephase(r.vis[v], phasor, r.w[v]);
+        } else if (c.inside) {
+          double2 W[N];
+          degrid_norms<N>(geo, g, wsum, mu, pos_w, cfcube, p.fxi, p.fyi, W);
+          for (int v = 0; v < N; ++v) {
+            double2 d = W[v];
+            if (d.x == 0.0 && d.y == 0.0) d = make_double2(1.0, 0.0);
+            d.y = -d.y;
+            gvis[i * N + v] = narrow(kdiv(cmul(residuals[i * N + v], phasor), d));
+          }
+        }
+      }
+      if (c.inside && update_weights) {
+        // twgt = sum_v |sum_XY cf'| * w_v (src/hpg_core.hpp:877-885)
+        counted = true;
+        for (int gpol = 0; gpol < geo.nr; ++gpol) {
+          const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
+          double t = 0.0;
+          for (int v = 0; v < N; ++v) {
+            const int m = row[v];
+            double mag = 0.0;
+            if (m >= 0 && m < g.nmueller) {
+              const double2 s = wsum[g.wsum_offset
+                                     + ((long long)(cfcube * g.nmueller + m) * geo.nf
+                                        + p.fxi) * geo.nf + p.fyi];
+              mag = hypot(s.x, s.y);
+            }
+            t += mag * double(r.w[v]);
+          }
+          tw[gpol] = t;
+        }
+      }
+      if (c.inside && do_grid) {
+        const int ntiles = geo.ntx * geo.nty;
+        const int tx0 = tiles_begin(c.gx), tx1 = tiles_end(c.gx, g.sx);
+        const int ty0 = tiles_begin(c.gy), ty1 = tiles_end(c.gy, g.sy);
+        for (int gpol = 0; gpol < geo.nr; ++gpol) {
+          if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
+          const size_t base = size_t(gcube * geo.nr + gpol) * ntiles;
+          for (int ty = ty0; ty < ty1; ++ty)
+            for (int tx = tx0; tx < tx1; ++tx)
+              atomicAdd(hist + base + ty * geo.ntx + tx, 1u);
+        }
+      }
+    }
+    prep[i] = p;
+  }
+
+  // weights: one FP64 atomic per (wave, grid cube, gpol)
+  if (update_weights) {
+    unsigned long long active = __ballot(counted);
+    const int lane = threadIdx.x % wave;
+    while (active) {
+      const int leader = __ffsll((long long)active) - 1;
+      const unsigned cube = __shfl(gcube, leader, wave);
+      const bool mine = counted && gcube == cube;
+      active &= ~__ballot(mine);
+      for (int gpol = 0; gpol < geo.nr; ++gpol) {
+        const double s = wave_sum(mine ? tw[gpol] : 0.0);
+        if (lane == leader) atomicAdd(weights + size_t(cube) * geo.nr + gpol, s);
+      }
+    }
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(block) void k_scatter(
+  size_t n, Geometry geo, const GroupInfo* __restrict__ groups, Mueller mu,
+  const PrepVis* __restrict__ prep, unsigned* __restrict__ cursor,
+  unsigned* __restrict__ entries) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const PrepVis p = prep[i];
+  if (!(p.flags & flag_inside)) return;
+  const GroupInfo g = groups[p.grp];
+  const bool pos_w = p.flags & flag_pos_w;
+  const int ntiles = geo.ntx * geo.nty;
+  const int tx0 = tiles_begin(p.gx), tx1 = tiles_end(p.gx, g.sx);
+  const int ty0 = tiles_begin(p.gy), ty1 = tiles_end(p.gy, g.sy);
+  for (int gpol = 0; gpol < geo.nr; ++gpol) {
+    if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
+    const size_t base = size_t(p.gcube * geo.nr + gpol) * ntiles;
+    for (int ty = ty0; ty < ty1; ++ty)
+      for (int tx = tx0; tx < tx1; ++tx) {
+        const unsigned pos = atomicAdd(cursor + base + ty * geo.ntx + tx, 1u);
+        entries[pos] = (unsigned(i) << 2) | unsigned(gpol);
+      }
+  }
+}
+
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// mode 0: complex<double> grid, FP64 accumulation of FP32 products
+// mode 1: complex<float> grid, FP32 products, two-sum compensated tiles
+template <int N, int MODE>
+__global__ __launch_bounds__(block) void k_tile(
+  Geometry geo, const GroupInfo* __restrict__ groups,
+  const float2* __restrict__ taps, Mueller mu,
+  const unsigned* __restrict__ offsets, const unsigned* __restrict__ entries,
+  const PrepVis* __restrict__ prep, const float2* __restrict__ gvis,
+  const VisRec<N>* __restrict__ recs, void* __restrict__ grid) {
+
+  using Acc = typename std::conditional<MODE == 0, double2, float4>::type;
+  __shared__ Acc acc[tile_waves * tile * tile];
+
+  const unsigned key = blockIdx.x;
+  const unsigned beg = offsets[key];
+  const unsigned end = offsets[key + 1];
+  if (beg == end) return;
+
+  const int ntiles = geo.ntx * geo.nty;
+  const unsigned plane = key / ntiles;
+  const int t = int(key % ntiles);
+  const int x0 = (t % geo.ntx) * tile;
+  const int y0 = (t / geo.ntx) * tile;
+  const int gpol = int(plane % geo.nr);
+
+  for (int k = threadIdx.x; k < tile_waves * tile * tile; k += block) {
+    if constexpr (MODE == 0)
+      acc[k] = make_double2(0.0, 0.0);
+    else
+      acc[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  __syncthreads();
+
+  const int w = threadIdx.x / wave;
+  const int lane = threadIdx.x % wave;
+  Acc* my = acc + w * tile * tile;
+
+  for (unsigned e = beg + w; e < end; e += tile_waves) {
+    const unsigned ent = uniform(int(entries[e]));
+    const unsigned i = ent >> 2;
+    const PrepVis p = prep[i];
+    const GroupInfo g = groups[p.grp];
+    const bool pos_w = p.flags & flag_pos_w;
+    const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
+    const float imf = pos_w ? -1.0f : 1.0f;
+
+    const int xa = max(p.gx, x0) - p.gx;
+    const int xb = min(p.gx + g.sx, x0 + tile) - p.gx;
+    const int ya = max(p.gy, y0) - p.gy;
+    const int yb = min(p.gy + g.sy, y0 + tile) - p.gy;
+    const int wdt = xb - xa;
+    const int ntap = wdt * (yb - ya);
+    const unsigned mg = magic(unsigned(wdt));
+
+    const float2* slice[N];
+    float2 gv[N];
+    bool use[N];
+    const long long slice_taps = (long long)g.sx * g.sy;
+    for (int v = 0; v < N; ++v) {
+      const int m = row[v];
+      use[v] = m >= 0 && m < g.nmueller;
+      slice[v] = taps + g.tap_offset
+        + (((long long)(p.cfcube * g.nmueller + (use[v] ? m : 0)) * geo.nf + p.fxi)
+             * geo.nf + p.fyi) * slice_taps;
+      gv[v] = gvis[size_t(i) * N + v];
+    }
+    const bool grad = p.flags & flag_gradient;
+    double phi0x = 0.0, phi0y = 0.0, dphix = 0.0, dphiy = 0.0;
+    if (grad) {
+      // Vis<N> phase screen origin and increment (src/hpg_core.hpp:543-547)
+      const double gxr = recs[i].grad[0];
+      const double gyr = recs[i].grad[1];
+      phi0x = -gxr * ((g.sx / 2) * geo.os - (p.fxi - geo.F));
+      phi0y = -gyr * ((g.sy / 2) * geo.os - (p.fyi - geo.F));
+      dphix = gxr * geo.os;
+      dphiy = gyr * geo.os;
+    }
+
+    for (int q = lane; q < ntap; q += wave) {
+      const int yl = int(div_magic(unsigned(q), unsigned(wdt), mg));
+      const int X = xa + (q - yl * wdt);
+      const int Y = ya + yl;
+      float2 scr = make_float2(1.0f, 0.0f);
+      if (grad) {
+        const double phX = phi0x + X * dphix;
+        const double phY = phi0y + Y * dphiy;
+        scr = narrow(cphase(phX + phY));
+      }
+      const int li = (p.gy + Y - y0) * tile + (p.gx + X - x0);
+      if constexpr (MODE == 0) {
+        double2 s = make_double2(0.0, 0.0);
+        for (int v = 0; v < N; ++v) {
+          if (!use[v]) continue;
+          float2 c = slice[v][Y * g.sx + X];
+          c.y *= imf;
+          if (grad) c = cmul(c, scr);
+          const float2 pr = cmul(c, gv[v]);
+          s.x += double(pr.x);
+          s.y += double(pr.y);
+        }
+        double2 a = my[li];
+        a.x += s.x;
+        a.y += s.y;
+        my[li] = a;
+      } else {
+        float4 a = my[li];
+        for (int v = 0; v < N; ++v) {
+          if (!use[v]) continue;
+          float2 c = slice[v][Y * g.sx + X];
+          c.y *= imf;
+          if (grad) c = cmul(c, scr);
+          const float2 pr = cmul(c, gv[v]);
+          float s, err;
+          two_sum(a.x, pr.x, s, err);
+          a.x = s;
+          a.z += err;
+          two_sum(a.y, pr.y, s, err);
+          a.y = s;
+          a.w += err;
+        }
+        my[li] = a;
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int k = threadIdx.x; k < tile * tile; k += block) {
+    const int xx = x0 + k % tile;
+    const int yy = y0 + k / tile;
+    if (xx >= geo.nx || yy >= geo.ny) continue;
+    const size_t gi = (size_t(plane) * geo.ny + yy) * geo.nx + xx;
+    if constexpr (MODE == 0) {
+      double2 s = acc[k];
+      for (int ww = 1; ww < tile_waves; ++ww) {
+        s.x += acc[ww * tile * tile + k].x;
+        s.y += acc[ww * tile * tile + k].y;
+      }
+      double2* gp = static_cast<double2*>(grid) + gi;
+      double2 gvl = *gp;
+      gvl.x += s.x;
+      gvl.y += s.y;
+      *gp = gvl;
+    } else {
+      float2* gp = static_cast<float2*>(grid) + gi;
+      const float2 g0 = *gp;
+      float sr = g0.x, si = g0.y, cr = 0.0f, ci = 0.0f;
+      for (int ww = 0; ww < tile_waves; ++ww) {
+        const float4 a = acc[ww * tile * tile + k];
+        float s, err;
+        two_sum(sr, a.x, s, err);
+        sr = s;
+        cr += err + a.z;
+        two_sum(si, a.y, s, err);
+        si = s;
+        ci += err + a.w;
+      }
+      *gp = make_float2(sr + cr, si + ci);
+    }
+  }
+}
+
+// ---- degridding --------------------------------------------------------
+
+template <int N>
+__global__ __launch_bounds__(block) void k_degrid(
+  size_t n, Geometry geo, const GroupInfo* __restrict__ groups,
+  const float2* __restrict__ taps, const double2* __restrict__ wsum, Mueller mu,
+  const double2* __restrict__ model, int do_grid, VisRec<N>* __restrict__ recs,
+  float2* __restrict__ gvis) {
+
+  const size_t i = blockIdx.x * (size_t)(block / wave) + threadIdx.x / wave;
+  const int lane = threadIdx.x % wave;
+  if (i >= n) return;
+  VisRec<N>& r = recs[i];
+  const unsigned grp = r.cf_index[1];
+  const unsigned cfcube = r.cf_index[0];
+  const unsigned gcube = r.grid_cube;
+  if (!(grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc))) return;
+  if (!(cfcube < unsigned(groups[grp].ncube))) return;
+  const GroupInfo g = groups[grp];
+  const Coord c = vis_coord(r, geo, g);
+  if (!c.inside) return;  // values left unchanged (src/hpg_core.hpp:1136-1169)
+  const bool pos_w = r.uvw[2] > 0;
+  const double2 phasor = cphase(r.phase);
+  const int fxi = c.fx + geo.F;
+  const int fyi = c.fy + geo.F;
+
+  float2 pred[N];
+  for (int v = 0; v < N; ++v) pred[v] = make_float2(0.0f, 0.0f);
+  if (model) {
+    const float imf = pos_w ? 1.0f : -1.0f;
+    const bool grad = r.grad[0] != 0.0 || r.grad[1] != 0.0;
+    const double phi0x = -r.grad[0] * ((g.sx / 2) * geo.os - c.fx);
+    const double phi0y = -r.grad[1] * ((g.sy / 2) * geo.os - c.fy);
+    const double dphix = r.grad[0] * geo.os;
+    const double dphiy = r.grad[1] * geo.os;
+    const int ntap = g.sx * g.sy;
+    const unsigned mg = magic(unsigned(g.sx));
+    const long long slice_taps = ntap;
+    double2 a[N];
+    for (int v = 0; v < N; ++v) a[v] = make_double2(0.0, 0.0);
+    for (int gpol = 0; gpol < geo.nr; ++gpol) {
+      const int* row = pos_w ? mu.Mc[gpol] : mu.M[gpol];
+      const float2* slice[N];
+      bool use[N];
+      for (int v = 0; v < N; ++v) {
+        const int m = row[v];
+        use[v] = m >= 0 && m < g.nmueller;
+        slice[v] = taps + g.tap_offset
+          + (((long long)(cfcube * g.nmueller + (use[v] ? m : 0)) * geo.nf + fxi)
+               * geo.nf + fyi) * slice_taps;
+      }
+      const double2* mplane =
+        model + (size_t(gcube) * geo.nr + gpol) * size_t(geo.ny) * geo.nx;
+      for (int q = lane; q < ntap; q += wave) {
+        const int Y = int(div_magic(unsigned(q), unsigned(g.sx), mg));
+        const int X = q - Y * g.sx;
+        double2 mv = mplane[size_t(c.gy + Y) * geo.nx + (c.gx + X)];
+        if (grad) {
+          const double phX = phi0x + X * dphix;
+          const double phY = phi0y + Y * dphiy;
+          mv = cmul(mv, cphase(-phX - phY));
+        }
+        for (int v = 0; v < N; ++v) {
+          if (!use[v]) continue;
+          float2 cf = slice[v][q];
+          cf.y *= imf;
+          const double2 p = cmul(cf, mv);
+          a[v].x += p.x;
+          a[v].y += p.y;
+        }
+      }
+    }
+    double2 W[N];
+    degrid_norms<N>(geo, g, wsum, mu, pos_w, cfcube, fxi, fyi, W);
+    const double2 conj_phasor = make_double2(phasor.x, -phasor.y);
+    for (int v = 0; v < N; ++v) {
+      const double2 s = make_double2(wave_sum(a[v].x), wave_sum(a[v].y));
+      const double2 d = (W[v].x != 0.0 || W[v].y != 0.0) ? W[v] : make_double2(1.0, 0.0);
+      pred[v] = narrow(cmul(kdiv(s, d), conj_phasor));
+    }
+  }
+  if (lane == 0) {
+    for (int v = 0; v < N; ++v) {
+      if (do_grid) {
+        float2 val = r.vis[v];
+        val.x -= pred[v].x;
+        val.y -= pred[v].y;
+        r.vis[v] = val;
+        gvis[i * N + v] = prephase(val, phasor, r.w[v]);
+      } else {
+        r.vis[v] = pred[v];
+      }
+    }
+  }
+}
+
+// ---- finishing passes ------------------------------------------------------
+
+template <typename T>
+__global__ void k_normalize(
+  T* __restrict__ grid, const double* __restrict__ weights, double wfactor,
+  size_t plane_size, size_t count) {
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const double d = wfactor * weights[k / plane_size];
+  T g = grid[k];
+  g.x = double(g.x) / d;
+  g.y = double(g.y) / d;
+  grid[k] = g;
+}
+
+template <typename T>
+__global__ void k_scale(T* __restrict__ grid, double s, size_t count) {
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  T g = grid[k];
+  g.x = double(g.x) * s;
+  g.y = double(g.y) * s;
+  grid[k] = g;
+}
+
+// even sizes: FORWARD and BACKWARD both swap (x, y) with
+// ((x + nx/2) mod nx, y + ny/2) -- one thread per pair
+template <typename T>
+__global__ void k_shift_even(T* __restrict__ grid, int nx, int ny, size_t nplanes) {
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t half = size_t(nx) * (ny / 2);
+  if (k >= half * nplanes) return;
+  const size_t p = k / half;
+  const int r = int(k % half);
+  const int x = r % nx;
+  const int y = r / nx;
+  T* plane = grid + p * size_t(nx) * ny;
+  const size_t a = size_t(y) * nx + x;
+  const size_t b = size_t(y + ny / 2) * nx + (x + nx / 2) % nx;
+  const T t = plane[a];
+  plane[a] = plane[b];
+  plane[b] = t;
+}
+
+template <typename T>
+__global__ void k_shift_general(
+  const T* __restrict__ src, T* __restrict__ dst, int nx, int ny, int sx, int sy,
+  size_t count) {
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const size_t ps = size_t(nx) * ny;
+  const size_t p = k / ps;
+  const int r = int(k % ps);
+  const int x = r % nx;
+  const int y = r / nx;
+  dst[p * ps + size_t((y + sy) % ny) * nx + (x + sx) % nx] = src[k];
+}
+
+__global__ void k_widen(const float2* __restrict__ s, double2* __restrict__ d, size_t n) {
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (k < n) d[k] = make_double2(double(s[k].x), double(s[k].y));
+}
+__global__ void k_narrow(const double2* __restrict__ s, float2* __restrict__ d, size_t n) {
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (k < n) d[k] = make_float2(float(s[k].x), float(s[k].y));
+}
+
+inline unsigned blocks_for(size_t n, int b = block) {
+  return unsigned((n + b - 1) / b);
+}
+
+}  // namespace
+
+// ---- launchers -------------------------------------------------------------
+
+hipError_t launch_cf_relayout(
+  hipStream_t s, const float2* staging, const unsigned ext[4],
+  const GroupInfo& gi, int os, int F, int nf, float2* taps) {
+  const long long total =
+    (long long)gi.ncube * gi.nmueller * nf * nf * gi.sx * gi.sy;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(
+    k_cf_relayout, dim3(blocks_for(size_t(total))), dim3(block), 0, s, staging,
+    ext[0], ext[1], gi, os, F, nf, taps, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_cf_wsum(
+  hipStream_t s, const GroupInfo& gi, int nf, const float2* taps,
+  double2* wsum) {
+  const long long ns = (long long)gi.ncube * gi.nmueller * nf * nf;
+  if (ns == 0) return hipSuccess;
+  hipLaunchKernelGGL(
+    k_cf_wsum, dim3(blocks_for(size_t(ns), 64)), dim3(64), 0, s, gi, nf, taps,
+    wsum, ns);
+  return hipGetLastError();
+}
+
+size_t tiles_per_footprint(int max_sx, int max_sy) {
+  const size_t tx = size_t((max_sx - 1 + tile - 1) / tile + 1);
+  const size_t ty = size_t((max_sy - 1 + tile - 1) / tile + 1);
+  return tx * ty;
+}
+
+hipError_t launch_prep(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, bool update_weights, bool do_grid, bool gvis_ready,
+  GvisSource src, const float2* residuals, CallBuffers& cb, double* weights) {
+  if (n == 0) return hipSuccess;
+  const int adj = src == GvisSource::Adjoint;
+#define HPGX_PREP(NP)                                                          \
+  hipLaunchKernelGGL(                                                          \
+    k_prep<NP>, dim3(blocks_for(n)), dim3(block), 0, s,                        \
+    static_cast<const VisRec<NP>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,    \
+    int(update_weights), int(do_grid), int(gvis_ready), adj, residuals,        \
+    cb.prep, cb.gvis, cb.hist, weights, cb.invalid_count)
+  switch (npol) {
+  case 1: HPGX_PREP(1); break;
+  case 2: HPGX_PREP(2); break;
+  case 3: HPGX_PREP(3); break;
+  case 4: HPGX_PREP(4); break;
+  default: return hipErrorInvalidValue;
+  }
+#undef HPGX_PREP
+  return hipGetLastError();
+}
+
+size_t scan_temp_bytes(size_t nkeys) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(
+    nullptr, bytes, static_cast<unsigned*>(nullptr),
+    static_cast<unsigned*>(nullptr), int(nkeys + 1));
+  return bytes;
+}
+
+hipError_t launch_scan(hipStream_t s, size_t nkeys, CallBuffers& cb) {
+  size_t bytes = cb.scan_temp_bytes;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(
+    cb.scan_temp, bytes, cb.hist, cb.offsets, int(nkeys + 1), s);
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(
+    cb.cursor, cb.offsets, nkeys * sizeof(unsigned), hipMemcpyDeviceToDevice, s);
+}
+
+hipError_t launch_scatter(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const Mueller& mu,
+  CallBuffers& cb, const GroupInfo* groups) {
+  if (n == 0) return hipSuccess;
+#define HPGX_SCATTER(NP)                                                       \
+  hipLaunchKernelGGL(                                                          \
+    k_scatter<NP>, dim3(blocks_for(n)), dim3(block), 0, s, n, geo, groups, mu, \
+    cb.prep, cb.cursor, cb.entries)
+  switch (npol) {
+  case 1: HPGX_SCATTER(1); break;
+  case 2: HPGX_SCATTER(2); break;
+  case 3: HPGX_SCATTER(3); break;
+  case 4: HPGX_SCATTER(4); break;
+  default: return hipErrorInvalidValue;
+  }
+#undef HPGX_SCATTER
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_grid(
+  hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
+  const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid) {
+  if (nkeys == 0) return hipSuccess;
+#define HPGX_TILE(NP, MD)                                                      \
+  hipLaunchKernelGGL(                                                          \
+    (k_tile<NP, MD>), dim3(unsigned(nkeys)), dim3(block), 0, s, geo,           \
+    cf.groups, cf.taps, mu, cb.offsets, cb.entries, cb.prep, cb.gvis,          \
+    static_cast<const VisRec<NP>*>(cb.vis), grid)
+  if (mode == 0) {
+    switch (npol) {
+    case 1: HPGX_TILE(1, 0); break;
+    case 2: HPGX_TILE(2, 0); break;
+    case 3: HPGX_TILE(3, 0); break;
+    case 4: HPGX_TILE(4, 0); break;
+    default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (npol) {
+    case 1: HPGX_TILE(1, 1); break;
+    case 2: HPGX_TILE(2, 1); break;
+    case 3: HPGX_TILE(3, 1); break;
+    case 4: HPGX_TILE(4, 1); break;
+    default: return hipErrorInvalidValue;
+    }
+  }
+#undef HPGX_TILE
+  return hipGetLastError();
+}
+
+hipError_t launch_degrid(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, const double2* model, bool do_grid, CallBuffers& cb) {
+  if (n == 0) return hipSuccess;
+  const unsigned nb = unsigned((n + (block / wave) - 1) / (block / wave));
+#define HPGX_DEGRID(NP)                                                        \
+  hipLaunchKernelGGL(                                                          \
+    k_degrid<NP>, dim3(nb), dim3(block), 0, s, n, geo, cf.groups, cf.taps,     \
+    cf.wsum, mu, model, int(do_grid), static_cast<VisRec<NP>*>(cb.vis),        \
+    cb.gvis)
+  switch (npol) {
+  case 1: HPGX_DEGRID(1); break;
+  case 2: HPGX_DEGRID(2); break;
+  case 3: HPGX_DEGRID(3); break;
+  case 4: HPGX_DEGRID(4); break;
+  default: return hipErrorInvalidValue;
+  }
+#undef HPGX_DEGRID
+  return hipGetLastError();
+}
+
+hipError_t launch_normalize(
+  hipStream_t s, int mode, void* grid, const double* weights, double wfactor,
+  const Geometry& geo) {
+  const size_t ps = size_t(geo.nx) * geo.ny;
+  const size_t count = ps * geo.nr * geo.nc;
+  if (count == 0) return hipSuccess;
+  if (mode == 0)
+    hipLaunchKernelGGL(
+      k_normalize<double2>, dim3(blocks_for(count)), dim3(block), 0, s,
+      static_cast<double2*>(grid), weights, wfactor, ps, count);
+  else
+    hipLaunchKernelGGL(
+      k_normalize<float2>, dim3(blocks_for(count)), dim3(block), 0, s,
+      static_cast<float2*>(grid), weights, wfactor, ps, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale(
+  hipStream_t s, int mode, void* grid, double inv_norm, size_t count) {
+  if (count == 0) return hipSuccess;
+  if (mode == 0)
+    hipLaunchKernelGGL(
+      k_scale<double2>, dim3(blocks_for(count)), dim3(block), 0, s,
+      static_cast<double2*>(grid), inv_norm, count);
+  else
+    hipLaunchKernelGGL(
+      k_scale<float2>, dim3(blocks_for(count)), dim3(block), 0, s,
+      static_cast<float2*>(grid), inv_norm, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_shift(
+  hipStream_t s, int mode, void* grid, void* scratch, int direction,
+  const Geometry& geo) {
+  const size_t ps = size_t(geo.nx) * geo.ny;
+  const size_t nplanes = size_t(geo.nr) * geo.nc;
+  const size_t count = ps * nplanes;
+  if (count == 0) return hipSuccess;
+  const size_t esize = mode == 0 ? sizeof(double2) : sizeof(float2);
+  if (geo.nx % 2 == 0 && geo.ny % 2 == 0) {
+    const size_t pairs = size_t(geo.nx) * (geo.ny / 2) * nplanes;
+    if (mode == 0)
+      hipLaunchKernelGGL(
+        k_shift_even<double2>, dim3(blocks_for(pairs)), dim3(block), 0, s,
+        static_cast<double2*>(grid), geo.nx, geo.ny, nplanes);
+    else
+      hipLaunchKernelGGL(
+        k_shift_even<float2>, dim3(blocks_for(pairs)), dim3(block), 0, s,
+        static_cast<float2*>(grid), geo.nx, geo.ny, nplanes);
+    return hipGetLastError();
+  }
+  const int sx = direction == 0 ? geo.nx / 2 : geo.nx - geo.nx / 2;
+  const int sy = direction == 0 ? geo.ny / 2 : geo.ny - geo.ny / 2;
+  if (mode == 0)
+    hipLaunchKernelGGL(
+      k_shift_general<double2>, dim3(blocks_for(count)), dim3(block), 0, s,
+      static_cast<const double2*>(grid), static_cast<double2*>(scratch), geo.nx,
+      geo.ny, sx, sy, count);
+  else
+    hipLaunchKernelGGL(
+      k_shift_general<float2>, dim3(blocks_for(count)), dim3(block), 0, s,
+      static_cast<const float2*>(grid), static_cast<float2*>(scratch), geo.nx,
+      geo.ny, sx, sy, count);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(grid, scratch, count * esize, hipMemcpyDeviceToDevice, s);
+}
+
+hipError_t launch_widen(hipStream_t s, const float2* src, double2* dst, size_t count) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_widen, dim3(blocks_for(count)), dim3(block), 0, s, src, dst, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_narrow(hipStream_t s, const double2* src, float2* dst, size_t count) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_narrow, dim3(blocks_for(count)), dim3(block), 0, s, src, dst, count);
+  return hipGetLastError();
+}
+
+}  // namespace hpgx

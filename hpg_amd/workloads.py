@@ -1,0 +1,92 @@
+"""Synthetic inputs for the gridding workloads (fixed seeds, numpy only).
+
+``gaussian_cf`` / ``config1_visibilities`` build the 1024x1024 single-pol
+workload: 1,000,000 VisData<1> records with N(0,1) values, weights in
+(0.5, 1.5], u, v uniform within +-0.45 of the grid extent, w = 0, and one
+7x7-support separable Gaussian CF (sigma 1.5 px) at oversampling 20.
+``random_cf`` / ``random_visibilities`` build small general cases (several
+groups and cubes, Mueller matrices, phase gradients, both signs of w) for the
+parity tests.
+"""
+import numpy as np
+
+from . import CFArray, visdata_dtype
+
+C_LIGHT = 299792458.0
+PADDING = CFArray.padding
+
+
+def cf_extent(radius, oversampling):
+    """CF extent (oversampled units) of a support of radius `radius`, with
+    the CF padding on both edges: radii() of it returns `radius`."""
+    return (2 * radius + 1 + 2 * PADDING) * oversampling
+
+
+def gaussian_cf(radius=3, oversampling=20, sigma=1.5, nmueller=1, ncube=1):
+    """Separable Gaussian taper g(tx) g(ty), g(t) = exp(-t^2 / (2 sigma^2)),
+    t the distance in pixels from the centre tap; imaginary part 0."""
+    ext = cf_extent(radius, oversampling)
+    centre = (radius + PADDING) * oversampling
+    t = (np.arange(ext) - centre) / oversampling
+    g = np.exp(-t * t / (2.0 * sigma * sigma))
+    plane = np.outer(g, g).astype(np.complex64)  # [y][x]
+    vals = np.broadcast_to(plane, (ncube, nmueller, ext, ext)).copy()
+    return CFArray(oversampling, [vals])
+
+
+def config1_visibilities(num_vis=1_000_000, grid=1024, extent_frac=0.45, seed=20261202):
+    """VisData<1> records of the single-pol workload (grid_scale 1, frequency
+    c, so a record's grid position is u + grid/2)."""
+    rng = np.random.default_rng(seed)
+    rec = np.zeros(num_vis, dtype=visdata_dtype(1))
+    vals = rng.standard_normal((num_vis, 2), dtype=np.float32)
+    rec["visibilities"][:, 0] = vals[:, 0] + 1j * vals[:, 1]
+    rec["weights"][:, 0] = (1.5 - rng.random(num_vis, dtype=np.float32)).astype(np.float32)
+    lim = extent_frac * grid
+    rec["uvw"][:, 0] = rng.uniform(-lim, lim, num_vis)
+    rec["uvw"][:, 1] = rng.uniform(-lim, lim, num_vis)
+    rec["frequency"] = C_LIGHT
+    return rec
+
+
+def random_cf(oversampling, radii, nmueller=1, ncube=1, seed=1, hermitian=False):
+    """CF groups with uniform random complex taps in [-1, 1]^2."""
+    rng = np.random.default_rng(seed)
+    groups = []
+    for r in radii:
+        rx, ry = (r, r) if np.isscalar(r) else r
+        ex, ey = cf_extent(rx, oversampling), cf_extent(ry, oversampling)
+        v = rng.uniform(-1, 1, (ncube, nmueller, ey, ex, 2)).astype(np.float32)
+        groups.append((v[..., 0] + 1j * v[..., 1]).astype(np.complex64))
+    return CFArray(oversampling, groups)
+
+
+def random_visibilities(num_vis, npol, grid_size, grid_scale, cf, seed=2,
+                        gradient=False, phase=False, w_signs=True, margin=0,
+                        freq_scale=1.0):
+    """Records spread over the grid (some near or beyond the edges when
+    margin < 0), random groups and cubes, optional phase gradients/phases."""
+    rng = np.random.default_rng(seed)
+    rec = np.zeros(num_vis, dtype=visdata_dtype(npol))
+    v = rng.uniform(-1, 1, (num_vis, npol, 2)).astype(np.float32)
+    rec["visibilities"] = v[..., 0] + 1j * v[..., 1]
+    rec["weights"] = rng.uniform(0, 1, (num_vis, npol)).astype(np.float32)
+    freq = C_LIGHT * freq_scale
+    rec["frequency"] = freq
+    ng = cf.num_groups()
+    grp = rng.integers(0, ng, num_vis)
+    cubes = np.array([cf.extents(g)[3] for g in range(ng)])
+    rec["cf_index"][:, 1] = grp
+    rec["cf_index"][:, 0] = (rng.random(num_vis) * cubes[grp]).astype(np.uint32)
+    rec["grid_cube"] = rng.integers(0, grid_size[3], num_vis)
+    for d in range(2):
+        half = grid_size[d] / 2.0 - margin
+        pos = rng.uniform(-half, half, num_vis)
+        rec["uvw"][:, d] = pos / (grid_scale[d] * freq / C_LIGHT)
+    if w_signs:
+        rec["uvw"][:, 2] = rng.uniform(-1, 1, num_vis)
+    if gradient:
+        rec["cf_phase_gradient"] = rng.uniform(-3.141, 3.141, (num_vis, 2))
+    if phase:
+        rec["phase"] = rng.uniform(-np.pi, np.pi, num_vis)
+    return rec

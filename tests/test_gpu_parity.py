@@ -1,0 +1,329 @@
+"""Parity of the HIP device (through the C ABI) with the CPU oracle.
+
+Bars (stated per test):
+  * FP64-grid mode, no phase gradient: the per-tap FP32 products are the
+    reference's bit for bit; only the order of FP64 accumulation differs, so
+    grids agree to 1e-12 of max|grid| and single-visibility grids exactly.
+  * phase gradients: the device's FP64 sin/cos may round the FP32 screen
+    differently in rare taps: 1e-6 of max|grid|.
+  * FP32-compensated mode: e_max, e_rms <= 1e-6 against the FP64 oracle.
+  * degridded visibilities: FP64 sums rounded to FP32: 1e-6 relative.
+  * normalize / shift: exact; FFT (hipFFT vs the oracle's FP64 FFT): 1e-12.
+"""
+import numpy as np
+import pytest
+
+import refcases as rc
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_view(rec, oracle_mod):
+    return np.ascontiguousarray(rec).view(oracle_mod.visdata_dtype(rec.dtype["visibilities"].shape[0]))
+
+
+def make_pair(hpg, oracle_mod, grid_size, grid_scale, M, cf, mode=0, Mc=None,
+              max_batch=100000):
+    Mc = M if Mc is None else Mc
+    gs = hpg.GridderState.create(hpg.Device.Hip, 0, max_batch, cf, grid_size,
+                                 grid_scale, M, Mc, grid_mode=mode)
+    gs.set_convolution_function(hpg.Device.Serial, cf)
+    o = oracle_mod.Oracle(grid_size, grid_scale, M, Mc)
+    o.set_cf(cf.oversampling(),
+             [(cf.extents(g), cf.values(g)) for g in range(cf.num_groups())])
+    return gs, o
+
+
+def rel_errors(got, ref):
+    scale = np.abs(ref).max()
+    d = np.abs(got - ref)
+    return d.max() / scale, np.linalg.norm(d) / np.linalg.norm(ref)
+
+
+def cone_cfarray(hpg, oracle_mod, nmueller, os_, radius):
+    _, vals = oracle_mod.cone_cf(nmueller, os_, radius)
+    return hpg.CFArray(os_, [vals])
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gridone(hpg, oracle_mod, gridone_golden, mode):
+    """GridOne on a 16384^2 grid: bit-exact with the oracle (FP64 mode, and
+    FP32-compensated mode since each pixel holds one FP32 product)."""
+    g = rc.GRIDONE
+    cf = cone_cfarray(hpg, oracle_mod, 1, g["oversampling"], g["radius"])
+    gs, o = make_pair(hpg, oracle_mod, g["grid_size"], g["grid_scale"], [[0]], cf, mode)
+    rec = rc.one_record(hpg.visdata_dtype(1), g["vis"], g["weight"], g["freq"], g["uvw"])
+    gs.grid_visibilities(hpg.Device.Serial, rec, True)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    x0, y0 = g["ll"][0] - 8, g["ll"][1] - 8
+    got = gs.grid_window(x0, y0, 107, 107)
+    exp = o.grid_window(x0, y0, 107, 107)
+    assert np.array_equal(got, exp)
+    win = gs.grid_window(g["ll"][0], g["ll"][1], 91, 91).T
+    assert np.array_equal(win != 0, gridone_golden != 0)
+    assert (win[:90, :90] == gridone_golden[:90, :90]).mean() > 0.98
+    np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-14)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gridding_sweep(hpg, oracle_mod, mode):
+    """Gridding test (GridderStateUnitTests.cc:1273-1341), exact."""
+    g = rc.GRIDDING
+    cf = cone_cfarray(hpg, oracle_mod, 1, g["oversampling"], g["radius"])
+    fine = g["grid_scale"][0] * g["oversampling"]
+    gs = hpg.GridderState.create(hpg.Device.Hip, 0, 1, cf, g["grid_size"],
+                                 g["grid_scale"], [[0]], grid_mode=mode)
+    gs.set_convolution_function(hpg.Device.Serial, cf)
+    for i in range(g["oversampling"] + 1):
+        for j in range(g["oversampling"] + 1):
+            uvw = (i * fine, j * fine, 0.0)
+            gs.reset_grid()
+            rec = rc.one_record(hpg.visdata_dtype(1), g["vis"], g["weight"], g["freq"], uvw)
+            gs.grid_visibilities(hpg.Device.Serial, rec, True)
+            exp = rc.single_vis_expectation(cf.values(0)[0, 0], g["grid_size"],
+                                            g["grid_scale"], g["freq"], uvw, g["radius"],
+                                            g["oversampling"], g["vis"])
+            assert np.array_equal(gs.grid_values()[0, 0], exp), (i, j)
+
+
+CASES = [
+    # npol, grid_size, M, Mc, radii, nmueller, ncube, os
+    (1, (40, 36, 1, 1), [[0]], [[0]], [3], 1, 1, 10),
+    (2, (48, 40, 2, 2), [[0, -1], [-1, 0]], [[0, -1], [-1, 0]], [2, (4, 3)], 1, 2, 8),
+    (2, (64, 64, 2, 1), [[0, 1], [2, 3]], [[1, 0], [3, 2]], [1, 5], 4, 1, 6),
+    (4, (70, 66, 4, 2), [[0, 1, 2, 3], [4, 5, 6, 7], [8, 9, 10, 11], [12, 13, 14, 15]],
+     [[0, 1, 2, 3], [4, 5, 6, 7], [8, 9, 10, 11], [12, 13, 14, 15]], [2, 7, 17], 16, 2, 5),
+    (3, (50, 50, 2, 1), [[0, -1, 1], [2, 0, -1]], [[1, -1, 0], [-1, 2, 0]], [3, 3], 3, 1, 7),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("gradient", [False, True])
+def test_random_vs_oracle_fp64(hpg, oracle_mod, case, gradient):
+    from hpg_amd import workloads
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[case]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=10 + case)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (0.9, 1.1), M, cf, 0, Mc)
+    rec = workloads.random_visibilities(700, npol, gsz, (0.9, 1.1), cf, seed=20 + case,
+                                        gradient=gradient, phase=True, freq_scale=1.1)
+    gs.grid_visibilities(hpg.Device.Serial, rec, True)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    got, exp = gs.grid_values(), o.grid_values()
+    assert np.abs(exp).max() > 0
+    e_max, _ = rel_errors(got, exp)
+    assert e_max <= (1e-6 if gradient else 1e-12), e_max
+    np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_random_vs_oracle_fp32_compensated(hpg, oracle_mod, case):
+    from hpg_amd import workloads
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[case]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=10 + case)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (0.9, 1.1), M, cf, 1, Mc)
+    rec = workloads.random_visibilities(3000, npol, gsz, (0.9, 1.1), cf, seed=30 + case,
+                                        phase=True)
+    gs.grid_visibilities(hpg.Device.Serial, rec, True)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    got, exp = gs.grid_values(), o.grid_values()
+    for c in range(gsz[3]):
+        for r in range(gsz[2]):
+            if np.abs(exp[c, r]).max() == 0:
+                continue
+            e_max, e_rms = rel_errors(got[c, r], exp[c, r])
+            assert e_max <= 1e-6 and e_rms <= 1e-6, (c, r, e_max, e_rms)
+    np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
+
+
+def test_serialization_and_batches(hpg, oracle_mod):
+    """Serialization (GridderStateUnitTests.cc:1126-1218): the grid does not
+    depend on the order of the batches; gridding in batches equals one call."""
+    from hpg_amd import workloads
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[1]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=3)
+    a = workloads.random_visibilities(500, npol, gsz, (1.0, 1.0), cf, seed=4)
+    b = workloads.random_visibilities(500, npol, gsz, (1.0, 1.0), cf, seed=5)
+    gs1, o = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), M, cf, 0, Mc)
+    gs2, _ = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), M, cf, 0, Mc)
+    gs1.grid_visibilities(hpg.Device.Serial, a)
+    gs1.grid_visibilities(hpg.Device.Serial, b)
+    gs2.grid_visibilities(hpg.Device.Serial, b)
+    gs2.grid_visibilities(hpg.Device.Serial, a)
+    g1, g2 = gs1.grid_values(), gs2.grid_values()
+    np.testing.assert_allclose(g1, g2, rtol=0, atol=1e-12 * np.abs(g1).max())
+    np.testing.assert_allclose(gs1.grid_weights(), gs2.grid_weights(), rtol=1e-13)
+
+
+def test_degrid_and_residual_vs_oracle(hpg, oracle_mod):
+    from hpg_amd import workloads
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[3]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=7)
+    rec = workloads.random_visibilities(800, npol, gsz, (1.0, 1.0), cf, seed=8,
+                                        gradient=True, phase=True)
+    rng = np.random.default_rng(9)
+    shape = (gsz[3], gsz[2], gsz[1], gsz[0])
+    model = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), M, cf, 0, Mc)
+    gs.set_model(hpg.Device.Serial, model)
+    o.set_model(model)
+    pred = gs.degrid_get_predicted_visibilities(hpg.Device.Serial, rec)
+    pexp = o.grid_visibilities(oracle_view(rec, oracle_mod), False, do_degrid=True,
+                               do_grid=False)
+    p, pe = pred["visibilities"], pexp["vis"]
+    assert np.abs(pe).max() > 0
+    np.testing.assert_allclose(p, pe, rtol=1e-6, atol=1e-6 * np.abs(pe).max())
+    # residual + gridding of the residual
+    res = gs.degrid_grid_get_residual_visibilities(hpg.Device.Serial, rec)
+    rexp = o.grid_visibilities(oracle_view(rec, oracle_mod), True, do_degrid=True,
+                               do_grid=True)
+    np.testing.assert_allclose(res["visibilities"], rexp["vis"], rtol=1e-6,
+                               atol=1e-6 * np.abs(rexp["vis"]).max())
+    e_max, _ = rel_errors(gs.grid_values(), o.grid_values())
+    assert e_max <= 1e-5, e_max
+
+
+def test_residual_without_model_is_input(hpg, oracle_mod):
+    """ResidualVisibilities (GridderStateUnitTests.cc:1743-1815)."""
+    cf = cone_cfarray(hpg, oracle_mod, 1, 10, 3)
+    gs = hpg.GridderState.create(hpg.Device.Hip, 0, 1, cf, (20, 20, 2, 1),
+                                 (0.0476591, 0.0476591), [[0, -1], [-1, 0]])
+    gs.set_convolution_function(hpg.Device.Serial, cf)
+    rec = rc.one_record(hpg.visdata_dtype(2), [1 - 1j, -1 + 1j], [1.0, 0.5], 3.693e9,
+                        (0.0, 0.0, 0.1))
+    res = gs.degrid_grid_get_residual_visibilities(hpg.Device.Serial, rec)
+    assert res.tobytes() == rec.tobytes()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_finishing_passes_vs_oracle(hpg, oracle_mod, mode):
+    """normalize_by_weights -> FFT -> shift on a gridded plane set."""
+    from hpg_amd import workloads
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[1]
+    gsz = (64, 48, 2, 2)
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=12)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), M, cf, mode, Mc)
+    rec = workloads.random_visibilities(2000, npol, gsz, (1.0, 1.0), cf, seed=13)
+    gs.grid_visibilities(hpg.Device.Serial, rec)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    gs.normalize_by_weights(2.0)
+    o.normalize_by_weights(2.0)
+    tol = 1e-12 if mode == 0 else 1e-6
+    e1, _ = rel_errors(gs.grid_values(), o.grid_values())
+    assert e1 <= tol, e1
+    for sign in (hpg.FFTSign.POSITIVE, hpg.FFTSign.NEGATIVE):
+        gs.apply_grid_fft(norm=3.0, sign=sign)
+        o.apply_fft(norm=3.0, sign=sign.name)
+        e2, _ = rel_errors(gs.grid_values(), o.grid_values())
+        assert e2 <= (1e-12 if mode == 0 else 2e-6), (sign, e2)
+    for d in (hpg.ShiftDirection.FORWARD, hpg.ShiftDirection.BACKWARD):
+        gs.shift_grid(d)
+        o.shift(d.name)
+        e3, _ = rel_errors(gs.grid_values(), o.grid_values())
+        assert e3 <= (1e-12 if mode == 0 else 2e-6), (d, e3)
+
+
+@pytest.mark.parametrize("shape", [(20, 20), (19, 19), (18, 19), (19, 18)])
+def test_shift_cycle(hpg, shape):
+    """ShiftCycle (GridderStateUnitTests.cc:1860-1924) through the model."""
+    nx, ny = shape
+    gs = hpg.GridderState.create(hpg.Device.Hip, 0, 1, None, (nx, ny, 1, 2), (1.0, 1.0), [[0]])
+    rng = np.random.default_rng(1)
+    m = rng.standard_normal((2, 1, ny, nx)) + 1j * rng.standard_normal((2, 1, ny, nx))
+    gs.set_model(hpg.Device.Serial, m)
+    gs.shift_model(hpg.ShiftDirection.FORWARD)
+    assert np.array_equal(gs.model_values(), np.fft.fftshift(m, axes=(-2, -1)))
+    gs.shift_model(hpg.ShiftDirection.BACKWARD)
+    assert np.array_equal(gs.model_values(), m)
+
+
+def test_one_fft_sign(hpg):
+    """OneFFT (GridderStateUnitTests.cc:1704-1740)."""
+    gs = hpg.GridderState.create(hpg.Device.Hip, 0, 15, None, (2048, 2048, 1, 1),
+                                 (0.12, -0.34), [[0]])
+    m = np.zeros((1, 1, 2048, 2048), dtype=np.complex128)
+    m[0, 0, 1108, 1024] = 0.825
+    gs.set_model(hpg.Device.Serial, m)
+    gs.apply_model_fft()
+    gs.shift_model(hpg.ShiftDirection.FORWARD)
+    assert gs.model_values()[0, 0, 10, 1070].imag < 0
+
+
+def test_layouts_and_copy(hpg, oracle_mod):
+    from hpg_amd import workloads
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[1]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=14)
+    gs, _ = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), M, cf, 0, Mc)
+    rec = workloads.random_visibilities(300, npol, gsz, (1.0, 1.0), cf, seed=15)
+    gs.grid_visibilities(hpg.Device.Serial, rec)
+    left = gs.grid_values(hpg.Layout.Left)
+    right = gs.grid_values(hpg.Layout.Right)
+    assert np.array_equal(right, left.transpose(3, 2, 1, 0))
+    wl, wr = gs.grid_weights(hpg.Layout.Left), gs.grid_weights(hpg.Layout.Right)
+    assert np.array_equal(wr, wl.T)
+    cp = gs.copy()
+    gs.grid_visibilities(hpg.Device.Serial, rec)
+    assert np.array_equal(cp.grid_values(), left)
+    cp.grid_visibilities(hpg.Device.Serial, rec)
+    assert np.array_equal(cp.grid_values(), gs.grid_values())
+
+
+def test_error_paths(hpg, oracle_mod):
+    E = hpg.ErrorType
+    cf = cone_cfarray(hpg, oracle_mod, 1, 10, 3)
+    gs = hpg.GridderState.create(hpg.Device.Hip, 0, 2, cf, (20, 20, 1, 1), (1.0, 1.0), [[0]])
+    gs.set_convolution_function(hpg.Device.Serial, cf)
+    rec3 = np.zeros(3, dtype=hpg.visdata_dtype(1))
+    with pytest.raises(hpg.Error) as e:
+        gs.grid_visibilities(hpg.Device.Serial, rec3)
+    assert e.value.type == E.ExcessiveNumberVisibilities
+    with pytest.raises(hpg.Error) as e:
+        gs.grid_visibilities(hpg.Device.Serial, np.zeros(1, dtype=hpg.visdata_dtype(2)))
+    assert e.value.type == E.InvalidNumberPolarizations
+    with pytest.raises(hpg.Error) as e:
+        gs.grid_visibilities_base(hpg.Device.Serial, rec3[:1], True, True, False, False)
+    assert e.value.type == E.UpdateWeightsWithoutGridding
+    with pytest.raises(hpg.Error) as e:
+        gs.grid_visibilities(hpg.Device.Cuda, rec3[:1])
+    assert e.value.type == E.DisabledHostDevice
+    big = cone_cfarray(hpg, oracle_mod, 1, 10, 12)  # extent 291 > 20 * 10
+    with pytest.raises(hpg.Error) as e:
+        gs.set_convolution_function(hpg.Device.Serial, big)
+    assert e.value.type == E.CFSupportExceedsGrid
+    with pytest.raises(hpg.Error) as e:
+        gs.set_model(hpg.Device.Serial, np.zeros((1, 1, 20, 21), dtype=np.complex128))
+    assert e.value.type == E.InvalidModelGridSize
+    bad = rc.one_record(hpg.visdata_dtype(1), 1 + 0j, 1.0, rc.C_LIGHT, (0.0, 0.0, 0.0),
+                        cf_index=(0, 5))
+    gs.grid_visibilities(hpg.Device.Serial, bad)
+    with pytest.raises(hpg.Error) as e:
+        gs.fence()
+    assert e.value.type == E.OutOfBoundsCFIndex
+    gs.fence()
+    assert not gs.grid_values().any()
+
+
+def test_empty_inputs(hpg, oracle_mod):
+    """GridNone / EmptyCF (GridderStateUnitTests.cc:1465-1521, 1817-1857)."""
+    cf = cone_cfarray(hpg, oracle_mod, 1, 20, 45)
+    gs = hpg.GridderState.create(hpg.Device.Hip, 0, 1, cf, (1024, 1024, 1, 1),
+                                 (0.0476591, 0.0476591), [[0]])
+    gs.grid_visibilities(hpg.Device.Serial, np.zeros(0, dtype=hpg.visdata_dtype(1)))
+    gs.set_convolution_function(hpg.Device.Serial, cf)
+    gs.grid_visibilities(hpg.Device.Serial, np.zeros(0, dtype=hpg.visdata_dtype(1)))
+    assert not gs.grid_values().any() and not gs.grid_weights().any()
+
+
+def test_device_resident_records(hpg, oracle_mod):
+    """Records handed over as a torch GPU tensor give the same grid."""
+    import torch
+    from hpg_amd import workloads
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[2]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=16)
+    rec = workloads.random_visibilities(500, npol, gsz, (1.0, 1.0), cf, seed=17)
+    gs1, _ = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), M, cf, 0, Mc)
+    gs2, _ = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), M, cf, 0, Mc)
+    gs1.grid_visibilities(hpg.Device.Serial, rec)
+    dev = torch.from_numpy(rec.view(np.uint8).copy()).cuda()
+    gs2.grid_visibilities(hpg.Device.Serial, dev)
+    a, b = gs1.grid_values(), gs2.grid_values()
+    np.testing.assert_allclose(b, a, rtol=0, atol=1e-12 * np.abs(a).max())
