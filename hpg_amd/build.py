@@ -43,7 +43,7 @@ def build(verbose=False):
         _run([HIPCC, *COMMON, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs,
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-soname,libhpgx.so", *objs,
           "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp])
     os.replace(tmp, LIB)
     if verbose:

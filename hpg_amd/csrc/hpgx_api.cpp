@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -69,6 +70,9 @@ struct hpgx_state_s {
 
   CallBuffers cb{};
   size_t vis_capacity = 0;  // records
+  // gridding pipeline: 2 = compact binned entry stream (default),
+  // 1 = per-visibility records gathered through an index list
+  int gridder = 2;
   size_t nkeys = 0;
 
   Plan plan_z2z, plan_c2c;
@@ -212,8 +216,15 @@ int ensure_call_buffers(hpgx_state_t st) {
       * hpgx::tiles_per_footprint(st->cf.max_sx, st->cf.max_sy);
     if (need > cb.entries_capacity) {
       dev_free(cb.entries);
+      dev_free(cb.stream_entries);
+      dev_free(cb.entry_vis);
       cb.entries_capacity = 0;
-      HPGX_HIP(st, dev_alloc(cb.entries, need * sizeof(unsigned)));
+      if (st->gridder == 1) {
+        HPGX_HIP(st, dev_alloc(cb.entries, need * sizeof(unsigned)));
+      } else {
+        HPGX_HIP(st, dev_alloc(cb.stream_entries, need * hpgx::entry_bytes(int(st->npol))));
+        HPGX_HIP(st, dev_alloc(cb.entry_vis, need * sizeof(unsigned)));
+      }
       cb.entries_capacity = need;
     }
   }
@@ -243,6 +254,8 @@ void free_state(hpgx_state_t st) {
   dev_free(cb.offsets);
   dev_free(cb.cursor);
   dev_free(cb.entries);
+  dev_free(cb.stream_entries);
+  dev_free(cb.entry_vis);
   dev_free(cb.scan_temp);
   dev_free(cb.invalid_count);
   if (st->plan_z2z.valid) hipfftDestroy(st->plan_z2z.h);
@@ -419,6 +432,19 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
   if (!do_grid && !update_weights) return HPGX_SUCCESS;
   if (do_grid)
     HPGX_HIP(st, hipMemsetAsync(cb.hist, 0, (st->nkeys + 1) * sizeof(unsigned), st->stream));
+  if (st->gridder != 1) {
+    HPGX_HIP(st, hpgx::launch_count(
+                   st->stream, np, n, geo, st->cf, st->mu, update_weights, do_grid, cb,
+                   st->weights));
+    if (!do_grid) return HPGX_SUCCESS;
+    HPGX_HIP(st, hpgx::launch_scan(st->stream, st->nkeys, cb));
+    HPGX_HIP(st, hpgx::launch_emit(
+                   st->stream, np, n, geo, st->cf, st->mu, gvis_ready, src, residuals, cb));
+    HPGX_HIP(st, hpgx::launch_tile_stream(
+                   st->stream, np, st->mode == HPGX_GRID_FP64 ? 0 : 1, st->nkeys, geo,
+                   st->cf, st->mu, cb, st->grid));
+    return HPGX_SUCCESS;
+  }
   HPGX_HIP(st, hpgx::launch_prep(
                  st->stream, np, n, geo, st->cf, st->mu, update_weights, do_grid,
                  gvis_ready, src, residuals, cb, st->weights));
@@ -525,6 +551,7 @@ int hpgx_create(
   st->gscale[1] = grid_scale[1];
   st->npol = num_polarizations;
   st->mode = grid_mode;
+  if (const char* g = std::getenv("HPGX_GRIDDER")) st->gridder = std::atoi(g) == 1 ? 1 : 2;
   // Mueller index matrices padded to 4x4 with -1
   // (copy_mueller_indexes, src/hpg_runtime.hpp:1590-1614)
   for (int r = 0; r < 4; ++r)
@@ -575,6 +602,7 @@ int hpgx_clone(hpgx_state_t src, hpgx_state_t* dst) {
     src->gsize[2], src->mueller_rows.data(), src->conj_mueller_rows.data(),
     src->mode, 0, 0, nullptr);
   if (rc) return fail(src, rc, "clone: " + std::string(g_create_error));
+  st->gridder = src->gridder;
   auto copy = [&](void* d, const void* s, size_t bytes) {
     return hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, st->stream);
   };
@@ -692,8 +720,8 @@ int hpgx_set_convolution_function(
   hpgx_state_t st, unsigned oversampling, unsigned num_groups,
   const unsigned* extents, const void* const* values, int memory) {
   if (!st) return HPGX_ERR_OTHER;
-  if (oversampling == 0)
-    return fail(st, HPGX_ERR_OTHER, "CF oversampling must be positive");
+  if (oversampling == 0 || oversampling > 253)
+    return fail(st, HPGX_ERR_OTHER, "CF oversampling must be in [1, 253]");
   if (num_groups > unsigned(hpgx::max_cf_groups))
     return fail(st, HPGX_ERR_OTHER, "too many CF groups");
   // CFSupportExceedsGrid check (src/hpg_runtime.hpp:892-902)
@@ -704,6 +732,8 @@ int hpgx_set_convolution_function(
       return fail(st, HPGX_ERR_CF_SUPPORT_EXCEEDS_GRID, "CF support size exceeds grid size");
     if (e[0] < 2 * hpgx::cf_padding * oversampling || e[1] < 2 * hpgx::cf_padding * oversampling)
       return fail(st, HPGX_ERR_OTHER, "CF extents smaller than the padding");
+    if (e[3] > 65536)
+      return fail(st, HPGX_ERR_OTHER, "at most 65536 CF cubes per group");
   }
   if (int rc = set_device(st)) return rc;
   // the previous CF may still be in use by queued kernels

@@ -81,6 +81,8 @@ struct CallBuffers {
   size_t entries_capacity;
   void* scan_temp;
   size_t scan_temp_bytes;
+  void* stream_entries;  // compact binned entries (Entry<N>), sorted by key
+  unsigned* entry_vis;   // record index of entries with a phase gradient
   unsigned long long* invalid_count;
 };
 
@@ -114,6 +116,18 @@ hipError_t launch_scatter(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const Mueller& mu,
   CallBuffers& cb, const GroupInfo* groups);
 hipError_t launch_tile_grid(
+  hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
+  const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid);
+hipError_t launch_count(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, bool update_weights, bool do_grid, CallBuffers& cb,
+  double* weights);
+size_t entry_bytes(int npol);
+hipError_t launch_emit(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, bool gvis_ready, GvisSource src, const float2* residuals,
+  CallBuffers& cb);
+hipError_t launch_tile_stream(
   hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid);
 hipError_t launch_degrid(

@@ -521,6 +521,322 @@ __global__ __launch_bounds__(block) void k_tile(
   }
 }
 
+// ---- gridding, binned-stream form ------------------------------------------
+//
+// k_count   one thread per visibility: skip rule, weights, bin counts
+// k_emit    one thread per visibility: writes one compact Entry per touched
+//           (plane, tile) bin -- support corner, fine offsets, CF group and
+//           cube, pre-phased values -- so the tile kernel reads its bin as one
+//           contiguous stream instead of gathering per-visibility records
+// k_tile2   one workgroup per bin, one shared LDS tile: waves load 64 entries
+//           at a time (one per lane), broadcast each to the whole wave with
+//           readlane, and accumulate with LDS atomics (FP64 adds, or FP32
+//           adds whose rounding error is recovered from the returned old
+//           value and accumulated in a second array -- compensated two-sum)
+
+struct EntryHead {
+  int gx, gy;
+  unsigned fpk;  // fxi | fyi << 8 | flags << 16
+  unsigned gpk;  // grp | cfcube << 16
+};
+template <int N>
+struct Entry {
+  EntryHead h;
+  float2 g[N];
+};
+
+template <int N>
+__global__ __launch_bounds__(block) void k_count(
+  const VisRec<N>* __restrict__ recs, size_t n, Geometry geo,
+  const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
+  Mueller mu, int update_weights, int do_grid, unsigned* __restrict__ hist,
+  double* __restrict__ weights, unsigned long long* __restrict__ invalid) {
+
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  bool counted = false;
+  unsigned gcube = 0;
+  double tw[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < n) {
+    const VisRec<N>& r = recs[i];
+    const unsigned grp = r.cf_index[1];
+    const unsigned cfcube = r.cf_index[0];
+    gcube = r.grid_cube;
+    bool valid = grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc);
+    if (valid) valid = cfcube < unsigned(groups[grp].ncube);
+    if (!valid) {
+      atomicAdd(invalid, 1ull);
+    } else {
+      const GroupInfo g = groups[grp];
+      const Coord c = vis_coord(r, geo, g);
+      const bool pos_w = r.uvw[2] > 0;
+      if (c.inside && update_weights) {
+        counted = true;
+        const int fxi = c.fx + geo.F, fyi = c.fy + geo.F;
+        for (int gpol = 0; gpol < geo.nr; ++gpol) {
+          const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
+          double t = 0.0;
+          for (int v = 0; v < N; ++v) {
+            const int m = row[v];
+            double mag = 0.0;
+            if (m >= 0 && m < g.nmueller) {
+              const double2 s = wsum[g.wsum_offset
+                                     + ((long long)(cfcube * g.nmueller + m) * geo.nf
+                                        + fxi) * geo.nf + fyi];
+              mag = hypot(s.x, s.y);
+            }
+            t += mag * double(r.w[v]);
+          }
+          tw[gpol] = t;
+        }
+      }
+      if (c.inside && do_grid) {
+        const int ntiles = geo.ntx * geo.nty;
+        const int tx0 = tiles_begin(c.gx), tx1 = tiles_end(c.gx, g.sx);
+        const int ty0 = tiles_begin(c.gy), ty1 = tiles_end(c.gy, g.sy);
+        for (int gpol = 0; gpol < geo.nr; ++gpol) {
+          if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
+          const size_t base = size_t(gcube * geo.nr + gpol) * ntiles;
+          for (int ty = ty0; ty < ty1; ++ty)
+            for (int tx = tx0; tx < tx1; ++tx)
+              atomicAdd(hist + base + ty * geo.ntx + tx, 1u);
+        }
+      }
+    }
+  }
+  if (update_weights) {
+    unsigned long long active = __ballot(counted);
+    const int lane = threadIdx.x % wave;
+    while (active) {
+      const int leader = __ffsll((long long)active) - 1;
+      const unsigned cube = __shfl(gcube, leader, wave);
+      const bool mine = counted && gcube == cube;
+      active &= ~__ballot(mine);
+      for (int gpol = 0; gpol < geo.nr; ++gpol) {
+        const double s = wave_sum(mine ? tw[gpol] : 0.0);
+        if (lane == leader) atomicAdd(weights + size_t(cube) * geo.nr + gpol, s);
+      }
+    }
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(block) void k_emit(
+  const VisRec<N>* __restrict__ recs, size_t n, Geometry geo,
+  const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
+  Mueller mu, int gvis_ready, int adjoint, const float2* __restrict__ gvis_in,
+  const float2* __restrict__ residuals, unsigned* __restrict__ cursor,
+  Entry<N>* __restrict__ entries, unsigned* __restrict__ entry_vis) {
+
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const VisRec<N>& r = recs[i];
+  const unsigned grp = r.cf_index[1];
+  const unsigned cfcube = r.cf_index[0];
+  const unsigned gcube = r.grid_cube;
+  if (!(grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc))) return;
+  const GroupInfo g = groups[grp];
+  if (!(cfcube < unsigned(g.ncube))) return;
+  const Coord c = vis_coord(r, geo, g);
+  if (!c.inside) return;
+  const bool pos_w = r.uvw[2] > 0;
+  const bool grad = r.grad[0] != 0.0 || r.grad[1] != 0.0;
+  const int fxi = c.fx + geo.F, fyi = c.fy + geo.F;
+
+  Entry<N> e;
+  e.h.gx = c.gx;
+  e.h.gy = c.gy;
+  e.h.fpk = unsigned(fxi) | (unsigned(fyi) << 8)
+    | (unsigned((pos_w ? flag_pos_w : 0) | (grad ? flag_gradient : 0)) << 16);
+  e.h.gpk = grp | (cfcube << 16);
+  if (gvis_ready) {
+    for (int v = 0; v < N; ++v) e.g[v] = gvis_in[i * N + v];
+  } else {
+    const double2 phasor = cphase(r.phase);
+    if (!adjoint) {
+      for (int v = 0; v < N; ++v) e.g[v] = prephase(r.vis[v], phasor, r.w[v]);
+    } else {
+      double2 W[N];
+      degrid_norms<N>(geo, g, wsum, mu, pos_w, cfcube, fxi, fyi, W);
+      for (int v = 0; v < N; ++v) {
+        double2 d = W[v];
+        if (d.x == 0.0 && d.y == 0.0) d = make_double2(1.0, 0.0);
+        d.y = -d.y;
+        e.g[v] = narrow(kdiv(cmul(residuals[i * N + v], phasor), d));
+      }
+    }
+  }
+  const int ntiles = geo.ntx * geo.nty;
+  const int tx0 = tiles_begin(c.gx), tx1 = tiles_end(c.gx, g.sx);
+  const int ty0 = tiles_begin(c.gy), ty1 = tiles_end(c.gy, g.sy);
+  for (int gpol = 0; gpol < geo.nr; ++gpol) {
+    if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
+    const size_t base = size_t(gcube * geo.nr + gpol) * ntiles;
+    for (int ty = ty0; ty < ty1; ++ty)
+      for (int tx = tx0; tx < tx1; ++tx) {
+        const unsigned pos = atomicAdd(cursor + base + ty * geo.ntx + tx, 1u);
+        entries[pos] = e;
+        if (grad) entry_vis[pos] = unsigned(i);
+      }
+  }
+}
+
+__device__ __forceinline__ int bcast(int v, int k) {
+  return __builtin_amdgcn_readlane(v, k);
+}
+__device__ __forceinline__ float bcast(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+template <int N, int MODE>
+__global__ __launch_bounds__(block) void k_tile2(
+  Geometry geo, const GroupInfo* __restrict__ groups,
+  const float2* __restrict__ taps, Mueller mu,
+  const unsigned* __restrict__ offsets, const Entry<N>* __restrict__ entries,
+  const unsigned* __restrict__ entry_vis, const VisRec<N>* __restrict__ recs,
+  void* __restrict__ grid) {
+
+  constexpr int TT = tile * tile;
+  // MODE 0: re[TT], im[TT] (double); MODE 1: hi_re, hi_im, lo_re, lo_im (float)
+  __shared__ double acc64[MODE == 0 ? 2 * TT : 1];
+  __shared__ float acc32[MODE == 1 ? 4 * TT : 1];
+
+  const unsigned key = blockIdx.x;
+  const unsigned beg = offsets[key];
+  const unsigned end = offsets[key + 1];
+  if (beg == end) return;
+
+  const int ntiles = geo.ntx * geo.nty;
+  const unsigned plane = key / ntiles;
+  const int t = int(key % ntiles);
+  const int x0 = (t % geo.ntx) * tile;
+  const int y0 = (t / geo.ntx) * tile;
+  const int gpol = int(plane % geo.nr);
+
+  if constexpr (MODE == 0) {
+    for (int k = threadIdx.x; k < 2 * TT; k += block) acc64[k] = 0.0;
+  } else {
+    for (int k = threadIdx.x; k < 4 * TT; k += block) acc32[k] = 0.0f;
+  }
+  __syncthreads();
+
+  const int w = threadIdx.x / wave;
+  const int lane = threadIdx.x % wave;
+
+  for (unsigned base = beg + w * wave; base < end; base += tile_waves * wave) {
+    const int cnt = int(min(unsigned(wave), end - base));
+    Entry<N> mine;
+    if (lane < cnt) mine = entries[base + lane];
+    for (int k = 0; k < cnt; ++k) {
+      const int gx = bcast(mine.h.gx, k);
+      const int gy = bcast(mine.h.gy, k);
+      const unsigned fpk = unsigned(bcast(int(mine.h.fpk), k));
+      const unsigned gpk = unsigned(bcast(int(mine.h.gpk), k));
+      float2 gv[N];
+      for (int v = 0; v < N; ++v)
+        gv[v] = make_float2(bcast(mine.g[v].x, k), bcast(mine.g[v].y, k));
+      const int fxi = int(fpk & 0xffu);
+      const int fyi = int((fpk >> 8) & 0xffu);
+      const unsigned flags = fpk >> 16;
+      const int grp = int(gpk & 0xffffu);
+      const int cfcube = int(gpk >> 16);
+      const GroupInfo g = groups[grp];
+      const bool pos_w = flags & flag_pos_w;
+      const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
+      const float imf = pos_w ? -1.0f : 1.0f;
+
+      const int xa = max(gx, x0) - gx;
+      const int xb = min(gx + g.sx, x0 + tile) - gx;
+      const int ya = max(gy, y0) - gy;
+      const int yb = min(gy + g.sy, y0 + tile) - gy;
+      const int wdt = xb - xa;
+      const int ntap = wdt * (yb - ya);
+      const unsigned mg = magic(unsigned(wdt));
+
+      const float2* slice[N];
+      bool use[N];
+      const long long slice_taps = (long long)g.sx * g.sy;
+      for (int v = 0; v < N; ++v) {
+        const int m = row[v];
+        use[v] = m >= 0 && m < g.nmueller;
+        slice[v] = taps + g.tap_offset
+          + (((long long)(cfcube * g.nmueller + (use[v] ? m : 0)) * geo.nf + fxi)
+               * geo.nf + fyi) * slice_taps;
+      }
+      const bool grad = flags & flag_gradient;
+      double phi0x = 0.0, phi0y = 0.0, dphix = 0.0, dphiy = 0.0;
+      if (grad) {
+        const unsigned i = entry_vis[base + k];
+        const double gxr = recs[i].grad[0];
+        const double gyr = recs[i].grad[1];
+        phi0x = -gxr * ((g.sx / 2) * geo.os - (fxi - geo.F));
+        phi0y = -gyr * ((g.sy / 2) * geo.os - (fyi - geo.F));
+        dphix = gxr * geo.os;
+        dphiy = gyr * geo.os;
+      }
+
+      for (int q = lane; q < ntap; q += wave) {
+        const int yl = int(div_magic(unsigned(q), unsigned(wdt), mg));
+        const int X = xa + (q - yl * wdt);
+        const int Y = ya + yl;
+        float2 scr = make_float2(1.0f, 0.0f);
+        if (grad) {
+          const double phX = phi0x + X * dphix;
+          const double phY = phi0y + Y * dphiy;
+          scr = narrow(cphase(phX + phY));
+        }
+        const int li = (gy + Y - y0) * tile + (gx + X - x0);
+        double sr = 0.0, si = 0.0;
+        for (int v = 0; v < N; ++v) {
+          if (!use[v]) continue;
+          float2 c = slice[v][Y * g.sx + X];
+          c.y *= imf;
+          if (grad) c = cmul(c, scr);
+          const float2 pr = cmul(c, gv[v]);
+          sr += double(pr.x);
+          si += double(pr.y);
+        }
+        if constexpr (MODE == 0) {
+          __hip_atomic_fetch_add(acc64 + li, sr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(acc64 + TT + li, si, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+          // split the FP64 tap sum into float hi + lo (exact for N == 1)
+          const float hr = float(sr), hi = float(si);
+          const float lr = float(sr - double(hr)), lim = float(si - double(hi));
+          const float orr = __hip_atomic_fetch_add(acc32 + li, hr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const float ori = __hip_atomic_fetch_add(acc32 + TT + li, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          float s, er, ei;
+          two_sum(orr, hr, s, er);
+          two_sum(ori, hi, s, ei);
+          __hip_atomic_fetch_add(acc32 + 2 * TT + li, er + lr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(acc32 + 3 * TT + li, ei + lim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int k = threadIdx.x; k < TT; k += block) {
+    const int xx = x0 + k % tile;
+    const int yy = y0 + k / tile;
+    if (xx >= geo.nx || yy >= geo.ny) continue;
+    const size_t gi = (size_t(plane) * geo.ny + yy) * geo.nx + xx;
+    if constexpr (MODE == 0) {
+      double2* gp = static_cast<double2*>(grid) + gi;
+      double2 gvl = *gp;
+      gvl.x += acc64[k];
+      gvl.y += acc64[TT + k];
+      *gp = gvl;
+    } else {
+      float2* gp = static_cast<float2*>(grid) + gi;
+      const float2 g0 = *gp;
+      float sr, er, si, ei;
+      two_sum(g0.x, acc32[k], sr, er);
+      two_sum(g0.y, acc32[TT + k], si, ei);
+      *gp = make_float2(sr + (er + acc32[2 * TT + k]), si + (ei + acc32[3 * TT + k]));
+    }
+  }
+}
+
 // ---- degridding --------------------------------------------------------
 
 template <int N>
@@ -807,6 +1123,83 @@ hipError_t launch_tile_grid(
     }
   }
 #undef HPGX_TILE
+  return hipGetLastError();
+}
+
+hipError_t launch_count(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, bool update_weights, bool do_grid, CallBuffers& cb,
+  double* weights) {
+  if (n == 0) return hipSuccess;
+#define HPGX_COUNT(NP)                                                         \
+  hipLaunchKernelGGL(                                                          \
+    k_count<NP>, dim3(blocks_for(n)), dim3(block), 0, s,                       \
+    static_cast<const VisRec<NP>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,    \
+    int(update_weights), int(do_grid), cb.hist, weights, cb.invalid_count)
+  switch (npol) {
+  case 1: HPGX_COUNT(1); break;
+  case 2: HPGX_COUNT(2); break;
+  case 3: HPGX_COUNT(3); break;
+  case 4: HPGX_COUNT(4); break;
+  default: return hipErrorInvalidValue;
+  }
+#undef HPGX_COUNT
+  return hipGetLastError();
+}
+
+size_t entry_bytes(int npol) { return sizeof(EntryHead) + sizeof(float2) * size_t(npol); }
+
+hipError_t launch_emit(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, bool gvis_ready, GvisSource src, const float2* residuals,
+  CallBuffers& cb) {
+  if (n == 0) return hipSuccess;
+  const int adj = src == GvisSource::Adjoint;
+#define HPGX_EMIT(NP)                                                          \
+  hipLaunchKernelGGL(                                                          \
+    k_emit<NP>, dim3(blocks_for(n)), dim3(block), 0, s,                        \
+    static_cast<const VisRec<NP>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,    \
+    int(gvis_ready), adj, cb.gvis, residuals, cb.cursor,                       \
+    static_cast<Entry<NP>*>(cb.stream_entries), cb.entry_vis)
+  switch (npol) {
+  case 1: HPGX_EMIT(1); break;
+  case 2: HPGX_EMIT(2); break;
+  case 3: HPGX_EMIT(3); break;
+  case 4: HPGX_EMIT(4); break;
+  default: return hipErrorInvalidValue;
+  }
+#undef HPGX_EMIT
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_stream(
+  hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
+  const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid) {
+  if (nkeys == 0) return hipSuccess;
+#define HPGX_TILE2(NP, MD)                                                     \
+  hipLaunchKernelGGL(                                                          \
+    (k_tile2<NP, MD>), dim3(unsigned(nkeys)), dim3(block), 0, s, geo,          \
+    cf.groups, cf.taps, mu, cb.offsets,                                        \
+    static_cast<const Entry<NP>*>(cb.stream_entries), cb.entry_vis,            \
+    static_cast<const VisRec<NP>*>(cb.vis), grid)
+  if (mode == 0) {
+    switch (npol) {
+    case 1: HPGX_TILE2(1, 0); break;
+    case 2: HPGX_TILE2(2, 0); break;
+    case 3: HPGX_TILE2(3, 0); break;
+    case 4: HPGX_TILE2(4, 0); break;
+    default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (npol) {
+    case 1: HPGX_TILE2(1, 1); break;
+    case 2: HPGX_TILE2(2, 1); break;
+    case 3: HPGX_TILE2(3, 1); break;
+    case 4: HPGX_TILE2(4, 1); break;
+    default: return hipErrorInvalidValue;
+    }
+  }
+#undef HPGX_TILE2
   return hipGetLastError();
 }
 
