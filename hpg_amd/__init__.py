@@ -412,22 +412,44 @@ class GridderState:
         return self.grid_visibilities_base(
             host_device, visibilities, False, True, True, False)
 
+    def degrid_forward(self, visibilities):
+        """Forward operator of the differentiable form (include/hpgx.h
+        hpgx_degrid_forward): FP64 predictions (n, npol) of the current model.
+        Returns a numpy complex128 array, or a torch complex128 GPU tensor
+        when the records are a torch GPU tensor."""
+        npol = self.num_polarizations()
+        ptr, n, memory, keep = _as_records(visibilities, npol)
+        if memory == 1:
+            import torch
+            out = torch.empty((n, npol), dtype=torch.complex128, device=keep.device)
+            optr = ctypes.c_void_p(out.data_ptr())
+        else:
+            out = np.empty((n, npol), dtype=np.complex128)
+            optr = ctypes.c_void_p(out.ctypes.data if n else 0)
+        _check(_lib.lib().hpgx_degrid_forward(self._h, ptr, optr, n, npol, memory), self._h)
+        if memory == 1:
+            self.fence()
+        return out
+
     def degrid_adjoint(self, visibilities, residuals):
-        """Adjoint of the degridding operator (include/hpgx.h
-        hpgx_degrid_adjoint): adds the gridding of r * phasor / conj(W) into
-        the grid.  residuals: complex64 (n, npol) (numpy or torch GPU)."""
+        """Adjoint of ``degrid_forward`` (include/hpgx.h hpgx_degrid_adjoint):
+        adds the gridding of r * phasor / conj(W) into the grid.  residuals:
+        complex128 (n, npol), numpy or torch GPU (with GPU records)."""
         npol = self.num_polarizations()
         ptr, n, memory, keep = _as_records(visibilities, npol)
         if hasattr(residuals, "data_ptr") and getattr(residuals, "is_cuda", False):
             if memory != 1:
                 raise ValueError("residuals and visibilities must be in the same memory")
-            rptr, rkeep = ctypes.c_void_p(residuals.data_ptr()), residuals
+            import torch
+            rkeep = residuals.to(torch.complex128).contiguous()
+            rptr = ctypes.c_void_p(rkeep.data_ptr())
         else:
             if memory != 0:
                 raise ValueError("residuals and visibilities must be in the same memory")
-            rkeep = np.ascontiguousarray(residuals, dtype=np.complex64)
+            rkeep = np.ascontiguousarray(residuals, dtype=np.complex128)
             rptr = ctypes.c_void_p(rkeep.ctypes.data if rkeep.size else 0)
         _check(_lib.lib().hpgx_degrid_adjoint(self._h, ptr, rptr, n, npol, memory), self._h)
+        self.fence()
         return self
 
     def fence(self):

@@ -190,6 +190,19 @@ int ensure_cf_capacity(hpgx_state_t st, size_t taps, size_t ws) {
   return HPGX_SUCCESS;
 }
 
+// bin entries one record can produce: one per grid row per touched tile
+size_t entries_per_record(hpgx_state_t st) {
+  return size_t(std::max(1u, st->gsize[2]))
+    * hpgx::tiles_per_footprint(std::max(1, st->cf.max_sx), std::max(1, st->cf.max_sy));
+}
+
+// device memory for bin entries (default 4 GiB; HPGX_ENTRY_BUDGET_MB)
+size_t entry_budget_bytes() {
+  const char* e = std::getenv("HPGX_ENTRY_BUDGET_MB");
+  const long mb = e ? std::atol(e) : 4096;
+  return size_t(mb > 0 ? mb : 4096) << 20;
+}
+
 // per-call buffers sized for max_batch records and the current CF supports
 int ensure_call_buffers(hpgx_state_t st) {
   CallBuffers& cb = st->cb;
@@ -212,8 +225,13 @@ int ensure_call_buffers(hpgx_state_t st) {
     HPGX_HIP(st, dev_alloc(cb.scan_temp, cb.scan_temp_bytes));
   }
   if (st->cf.ngroups > 0) {
-    const size_t need = st->vis_capacity * st->gsize[2]
-      * hpgx::tiles_per_footprint(st->cf.max_sx, st->cf.max_sy);
+    // bin entries for the whole batch, up to the entry memory budget; larger
+    // batches are gridded in chunks (run_grid)
+    const size_t epr = entries_per_record(st);
+    const size_t per_entry = st->gridder == 1
+      ? sizeof(unsigned) : hpgx::entry_bytes(int(st->npol)) + sizeof(unsigned);
+    const size_t budget = std::max(entry_budget_bytes() / per_entry, epr);
+    const size_t need = std::min(st->vis_capacity * epr, budget);
     if (need > cb.entries_capacity) {
       dev_free(cb.entries);
       dev_free(cb.stream_entries);
@@ -418,8 +436,10 @@ int upload_model(hpgx_state_t st, const void* values, int layout, int memory) {
 }
 
 // gridding / degridding pipeline on records already in cb.vis
+// gridding / degridding pipeline on records already in cb.vis; the records
+// are gridded in chunks whose bin entries fit the entry buffer
 int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
-             bool do_grid, hpgx::GvisSource src, const float2* residuals) {
+             bool do_grid, hpgx::GvisSource src, const double2* residuals) {
   const Geometry geo = st->geometry();
   CallBuffers& cb = st->cb;
   const int np = int(st->npol);
@@ -429,31 +449,38 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
                    st->stream, np, n, geo, st->cf, st->mu, st->model, do_grid, cb));
     gvis_ready = true;
   }
-  if (!do_grid && !update_weights) return HPGX_SUCCESS;
-  if (do_grid)
-    HPGX_HIP(st, hipMemsetAsync(cb.hist, 0, (st->nkeys + 1) * sizeof(unsigned), st->stream));
-  if (st->gridder != 1) {
-    HPGX_HIP(st, hpgx::launch_count(
-                   st->stream, np, n, geo, st->cf, st->mu, update_weights, do_grid, cb,
-                   st->weights));
-    if (!do_grid) return HPGX_SUCCESS;
-    HPGX_HIP(st, hpgx::launch_scan(st->stream, st->nkeys, cb));
-    HPGX_HIP(st, hpgx::launch_emit(
-                   st->stream, np, n, geo, st->cf, st->mu, gvis_ready, src, residuals, cb));
-    HPGX_HIP(st, hpgx::launch_tile_stream(
-                   st->stream, np, st->mode == HPGX_GRID_FP64 ? 0 : 1, st->nkeys, geo,
-                   st->cf, st->mu, cb, st->grid));
-    return HPGX_SUCCESS;
-  }
-  HPGX_HIP(st, hpgx::launch_prep(
-                 st->stream, np, n, geo, st->cf, st->mu, update_weights, do_grid,
-                 gvis_ready, src, residuals, cb, st->weights));
   if (!do_grid) return HPGX_SUCCESS;
-  HPGX_HIP(st, hpgx::launch_scan(st->stream, st->nkeys, cb));
-  HPGX_HIP(st, hpgx::launch_scatter(st->stream, np, n, geo, st->mu, cb, st->cf.groups));
-  HPGX_HIP(st, hpgx::launch_tile_grid(
-                 st->stream, np, st->mode == HPGX_GRID_FP64 ? 0 : 1, st->nkeys, geo,
-                 st->cf, st->mu, cb, st->grid));
+  const int mode = st->mode == HPGX_GRID_FP64 ? 0 : 1;
+  const size_t epr = entries_per_record(st);
+  const size_t chunk = std::max<size_t>(1, cb.entries_capacity / epr);
+  const size_t rec_bytes = visdata_size(st->npol);
+  for (size_t off = 0; off < n; off += chunk) {
+    const size_t m = std::min(chunk, n - off);
+    CallBuffers c = cb;
+    c.vis = static_cast<char*>(cb.vis) + off * rec_bytes;
+    c.prep = cb.prep + off;
+    c.gvis = cb.gvis + off * np;
+    const double2* res = residuals ? residuals + off * np : nullptr;
+    HPGX_HIP(st, hipMemsetAsync(c.hist, 0, (st->nkeys + 1) * sizeof(unsigned), st->stream));
+    if (st->gridder != 1) {
+      HPGX_HIP(st, hpgx::launch_count(
+                     st->stream, np, m, geo, st->cf, st->mu, update_weights, true, c,
+                     st->weights));
+      HPGX_HIP(st, hpgx::launch_scan(st->stream, st->nkeys, c));
+      HPGX_HIP(st, hpgx::launch_emit(
+                     st->stream, np, m, geo, st->cf, st->mu, gvis_ready, src, res, c));
+      HPGX_HIP(st, hpgx::launch_tile_stream(
+                     st->stream, np, mode, st->nkeys, geo, st->cf, st->mu, c, st->grid));
+    } else {
+      HPGX_HIP(st, hpgx::launch_prep(
+                     st->stream, np, m, geo, st->cf, st->mu, update_weights, true,
+                     gvis_ready, src, res, c, st->weights));
+      HPGX_HIP(st, hpgx::launch_scan(st->stream, st->nkeys, c));
+      HPGX_HIP(st, hpgx::launch_scatter(st->stream, np, m, geo, st->mu, c, st->cf.groups));
+      HPGX_HIP(st, hpgx::launch_tile_grid(
+                     st->stream, np, mode, st->nkeys, geo, st->cf, st->mu, c, st->grid));
+    }
+  }
   return HPGX_SUCCESS;
 }
 
@@ -884,13 +911,11 @@ int hpgx_degrid_adjoint(
   if (int rc = set_device(st)) return rc;
   if (int rc = ensure_call_buffers(st)) return rc;
   if (int rc = stage_records(st, vis, n, memory)) return rc;
-  // residuals staged into the gvis buffer's tail is not possible (gvis is the
-  // output); use a temporary device copy when they come from the host
-  const float2* res = static_cast<const float2*>(residuals);
-  float2* tmp = nullptr;
+  const double2* res = static_cast<const double2*>(residuals);
+  double2* tmp = nullptr;
   if (memory != HPGX_DEVICE) {
-    HPGX_HIP(st, dev_alloc(tmp, n * npol * sizeof(float2)));
-    HPGX_HIP(st, hipMemcpyAsync(tmp, residuals, n * npol * sizeof(float2),
+    HPGX_HIP(st, dev_alloc(tmp, n * npol * sizeof(double2)));
+    HPGX_HIP(st, hipMemcpyAsync(tmp, residuals, n * npol * sizeof(double2),
                                 hipMemcpyHostToDevice, st->stream));
     res = tmp;
   }
@@ -900,6 +925,42 @@ int hpgx_degrid_adjoint(
     dev_free(tmp);
   }
   return rc;
+}
+
+int hpgx_degrid_forward(
+  hpgx_state_t st, const void* vis, void* predictions, size_t n, unsigned npol,
+  int memory) {
+  if (!st) return HPGX_ERR_OTHER;
+  if (n > st->max_batch)
+    return fail(st, HPGX_ERR_EXCESSIVE_NUMBER_VISIBILITIES,
+                "Number of visibilities exceeds maximum batch size");
+  if (npol != st->npol)
+    return fail(st, HPGX_ERR_INVALID_NUMBER_POLARIZATIONS,
+                "Number of visibility polarizations does not match Mueller matrix");
+  if (n == 0) return HPGX_SUCCESS;
+  if (!vis || !predictions) return fail(st, HPGX_ERR_OTHER, "null argument");
+  if (int rc = set_device(st)) return rc;
+  const size_t bytes = n * npol * sizeof(double2);
+  double2* out = static_cast<double2*>(predictions);
+  double2* tmp = nullptr;
+  if (memory != HPGX_DEVICE) {
+    HPGX_HIP(st, dev_alloc(tmp, bytes));
+    out = tmp;
+  }
+  // records outside the grid (or with no CF) predict zero
+  HPGX_HIP(st, hipMemsetAsync(out, 0, bytes, st->stream));
+  if (st->cf.ngroups > 0) {
+    if (int rc = ensure_call_buffers(st)) return rc;
+    if (int rc = stage_records(st, vis, n, memory)) return rc;
+    HPGX_HIP(st, hpgx::launch_degrid(st->stream, int(npol), n, st->geometry(), st->cf,
+                                     st->mu, st->model, false, st->cb, out));
+  }
+  if (tmp) {
+    HPGX_HIP(st, hipMemcpyAsync(predictions, tmp, bytes, hipMemcpyDeviceToHost, st->stream));
+    HPGX_HIP(st, hipStreamSynchronize(st->stream));
+    dev_free(tmp);
+  }
+  return HPGX_SUCCESS;
 }
 
 int hpgx_fence(hpgx_state_t st) {

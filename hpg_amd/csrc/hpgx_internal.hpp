@@ -109,7 +109,7 @@ size_t tiles_per_footprint(int max_sx, int max_sy);
 hipError_t launch_prep(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool update_weights, bool do_grid, bool gvis_ready,
-  GvisSource src, const float2* residuals, CallBuffers& cb, double* weights);
+  GvisSource src, const double2* residuals, CallBuffers& cb, double* weights);
 hipError_t launch_scan(hipStream_t s, size_t nkeys, CallBuffers& cb);
 size_t scan_temp_bytes(size_t nkeys);
 hipError_t launch_scatter(
@@ -125,14 +125,15 @@ hipError_t launch_count(
 size_t entry_bytes(int npol);
 hipError_t launch_emit(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
-  const Mueller& mu, bool gvis_ready, GvisSource src, const float2* residuals,
+  const Mueller& mu, bool gvis_ready, GvisSource src, const double2* residuals,
   CallBuffers& cb);
 hipError_t launch_tile_stream(
   hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid);
 hipError_t launch_degrid(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
-  const Mueller& mu, const double2* model, bool do_grid, CallBuffers& cb);
+  const Mueller& mu, const double2* model, bool do_grid, CallBuffers& cb,
+  double2* pred64 = nullptr);
 
 hipError_t launch_normalize(
   hipStream_t s, int mode, void* grid, const double* weights, double wfactor,

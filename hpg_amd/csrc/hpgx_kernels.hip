@@ -224,7 +224,7 @@ __global__ __launch_bounds__(block) void k_prep(
   const VisRec<N>* __restrict__ recs, size_t n, Geometry geo,
   const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
   Mueller mu, int update_weights, int do_grid, int gvis_ready, int adjoint,
-  const float2* __restrict__ residuals, PrepVis* __restrict__ prep,
+  const double2* __restrict__ residuals, PrepVis* __restrict__ prep,
   float2* __restrict__ gvis, unsigned* __restrict__ hist,
   double* __restrict__ weights, unsigned long long* __restrict__ invalid) {
 
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(block) void k_emit(
   const VisRec<N>* __restrict__ recs, size_t n, Geometry geo,
   const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
   Mueller mu, int gvis_ready, int adjoint, const float2* __restrict__ gvis_in,
-  const float2* __restrict__ residuals, unsigned* __restrict__ cursor,
+  const double2* __restrict__ residuals, unsigned* __restrict__ cursor,
   Entry<N>* __restrict__ entries, unsigned* __restrict__ entry_vis) {
 
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -844,7 +844,7 @@ __global__ __launch_bounds__(block) void k_degrid(
   size_t n, Geometry geo, const GroupInfo* __restrict__ groups,
   const float2* __restrict__ taps, const double2* __restrict__ wsum, Mueller mu,
   const double2* __restrict__ model, int do_grid, VisRec<N>* __restrict__ recs,
-  float2* __restrict__ gvis) {
+  float2* __restrict__ gvis, double2* __restrict__ pred64) {
 
   const size_t i = blockIdx.x * (size_t)(block / wave) + threadIdx.x / wave;
   const int lane = threadIdx.x % wave;
@@ -864,7 +864,11 @@ __global__ __launch_bounds__(block) void k_degrid(
   const int fyi = c.fy + geo.F;
 
   float2 pred[N];
-  for (int v = 0; v < N; ++v) pred[v] = make_float2(0.0f, 0.0f);
+  double2 pred_d[N];
+  for (int v = 0; v < N; ++v) {
+    pred[v] = make_float2(0.0f, 0.0f);
+    pred_d[v] = make_double2(0.0, 0.0);
+  }
   if (model) {
     const float imf = pos_w ? 1.0f : -1.0f;
     const bool grad = r.grad[0] != 0.0 || r.grad[1] != 0.0;
@@ -915,8 +919,15 @@ __global__ __launch_bounds__(block) void k_degrid(
     for (int v = 0; v < N; ++v) {
       const double2 s = make_double2(wave_sum(a[v].x), wave_sum(a[v].y));
       const double2 d = (W[v].x != 0.0 || W[v].y != 0.0) ? W[v] : make_double2(1.0, 0.0);
-      pred[v] = narrow(cmul(kdiv(s, d), conj_phasor));
+      pred_d[v] = cmul(kdiv(s, d), conj_phasor);
+      pred[v] = narrow(pred_d[v]);
     }
+  }
+  if (pred64) {
+    // forward operator of the differentiable form: FP64 predictions
+    if (lane == 0)
+      for (int v = 0; v < N; ++v) pred64[i * N + v] = pred_d[v];
+    return;
   }
   if (lane == 0) {
     for (int v = 0; v < N; ++v) {
@@ -1040,7 +1051,7 @@ size_t tiles_per_footprint(int max_sx, int max_sy) {
 hipError_t launch_prep(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool update_weights, bool do_grid, bool gvis_ready,
-  GvisSource src, const float2* residuals, CallBuffers& cb, double* weights) {
+  GvisSource src, const double2* residuals, CallBuffers& cb, double* weights) {
   if (n == 0) return hipSuccess;
   const int adj = src == GvisSource::Adjoint;
 #define HPGX_PREP(NP)                                                          \
@@ -1151,7 +1162,7 @@ size_t entry_bytes(int npol) { return sizeof(EntryHead) + sizeof(float2) * size_
 
 hipError_t launch_emit(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
-  const Mueller& mu, bool gvis_ready, GvisSource src, const float2* residuals,
+  const Mueller& mu, bool gvis_ready, GvisSource src, const double2* residuals,
   CallBuffers& cb) {
   if (n == 0) return hipSuccess;
   const int adj = src == GvisSource::Adjoint;
@@ -1205,14 +1216,15 @@ hipError_t launch_tile_stream(
 
 hipError_t launch_degrid(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
-  const Mueller& mu, const double2* model, bool do_grid, CallBuffers& cb) {
+  const Mueller& mu, const double2* model, bool do_grid, CallBuffers& cb,
+  double2* pred64) {
   if (n == 0) return hipSuccess;
   const unsigned nb = unsigned((n + (block / wave) - 1) / (block / wave));
 #define HPGX_DEGRID(NP)                                                        \
   hipLaunchKernelGGL(                                                          \
     k_degrid<NP>, dim3(nb), dim3(block), 0, s, n, geo, cf.groups, cf.taps,     \
     cf.wsum, mu, model, int(do_grid), static_cast<VisRec<NP>*>(cb.vis),        \
-    cb.gvis)
+    cb.gvis, pred64)
   switch (npol) {
   case 1: HPGX_DEGRID(1); break;
   case 2: HPGX_DEGRID(2); break;

@@ -187,13 +187,23 @@ HPGX_API int hpgx_grid_visibilities(
   unsigned num_polarizations, int update_grid_weights, int do_degrid,
   int return_visibilities, int do_grid, void* visibilities_out, int memory);
 
-/* Adjoint of the degridding operator (model -> predicted visibilities) for
- * differentiable imaging: grids r_v * phasor / conj(W_v) with unit weight,
- * where r are `residuals` (num_visibilities x num_polarizations
- * complex<float>, in `memory`) and W_v the degridder's CF normalization, for
- * the records' coordinates.  Adds into the grid; grid weights unchanged.
- * Exact adjoint when the Mueller index matrices equal their conjugates.
- * No reference counterpart (the reference has no differentiable form). */
+/* Forward operator of the differentiable form: the degridded (predicted)
+ * visibilities of the current model for the records' coordinates, before
+ * their rounding to complex<float>: predictions are num_visibilities x
+ * num_polarizations complex<double>, in `memory`.  The records are not
+ * modified and the grid is untouched.  Same arithmetic as the degridding of
+ * hpgx_grid_visibilities (degrid_vis, src/hpg_core.hpp:579-682).  No
+ * reference counterpart (the reference has no differentiable form). */
+HPGX_API int hpgx_degrid_forward(
+  hpgx_state_t state, const void* visibilities, void* predictions,
+  size_t num_visibilities, unsigned num_polarizations, int memory);
+
+/* Adjoint of hpgx_degrid_forward (model -> predicted visibilities): grids
+ * r_v * phasor / conj(W_v) with unit weight, where r are `residuals`
+ * (num_visibilities x num_polarizations complex<double>, in `memory`) and
+ * W_v the degridder's CF normalization, for the records' coordinates.  Adds
+ * into the grid; grid weights unchanged.  Exact adjoint when the Mueller
+ * index matrices equal their conjugates.  No reference counterpart. */
 HPGX_API int hpgx_degrid_adjoint(
   hpgx_state_t state, const void* visibilities, const void* residuals,
   size_t num_visibilities, unsigned num_polarizations, int memory);
