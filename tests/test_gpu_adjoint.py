@@ -62,8 +62,12 @@ def test_forward_matches_degridder(hpg):
     gs.set_model(hpg.Device.Serial, model)
     am = gs.degrid_forward(rec)
     pred = gs.degrid_get_predicted_visibilities(hpg.Device.Serial, rec)["visibilities"]
-    assert np.array_equal(am.astype(np.complex64), pred.astype(np.complex64)) or \
-        np.abs(am - pred).max() <= 1e-6 * np.abs(am).max()
+    # records outside the grid: the operator predicts 0, the degridder leaves
+    # their values unchanged (src/hpg_core.hpp:1136-1169)
+    inside = np.any(am != 0, axis=1)
+    assert inside.sum() > 0.5 * rec.shape[0]
+    assert np.array_equal(pred[~inside], rec["visibilities"][~inside])
+    assert np.array_equal(am[inside].astype(np.complex64), pred[inside])
 
 
 def test_finite_differences_via_autograd(hpg):
