@@ -70,10 +70,15 @@ struct hpgx_state_s {
 
   CallBuffers cb{};
   size_t vis_capacity = 0;  // records
-  // gridding pipeline: 2 = compact binned entry stream (default),
-  // 1 = per-visibility records gathered through an index list
-  int gridder = 2;
+  // gridding pipeline: 4 = block-local bins of one entry per record and
+  // plane, haloed LDS tiles (default; falls back to 3 when a CF support
+  // reaches the tile edge, and to 2 when the grid has more (plane, tile) keys
+  // than one workgroup's LDS histogram holds), 3 = block-local bins of one
+  // entry per touched tile, 2 = compact entries binned through global
+  // counters, 1 = per-visibility records gathered through an index
+  int gridder = 3;
   size_t nkeys = 0;
+  size_t pass_blocks = 0;  // bin blocks per pass of the block-local pipelines
 
   Plan plan_z2z, plan_c2c;
   hipEvent_t events[num_event_slots] = {};
@@ -191,8 +196,37 @@ int ensure_cf_capacity(hpgx_state_t st, size_t taps, size_t ws) {
 }
 
 // bin entries one record can produce: one per grid row per touched tile
+// block-local bins: the pipelines of choice when one workgroup's LDS holds
+// a histogram over all (plane, tile) keys
+bool use_block_local(hpgx_state_t st) {
+  return (st->gridder == 3 || st->gridder == 4)
+    && st->nkeys <= size_t(hpgx::bin_keys_limit());
+}
+
+// halo widths of the haloed-tile gridder: largest CF support - 1 per axis
+int halo_x(hpgx_state_t st) { return std::max(1, st->cf.max_sx) - 1; }
+int halo_y(hpgx_state_t st) { return std::max(1, st->cf.max_sy) - 1; }
+
+// haloed tiles: supports narrower than a tile, tile + halo within LDS
+bool use_halo(hpgx_state_t st) {
+  if (st->gridder != 4 || !use_block_local(st)) return false;
+  const int hx = halo_x(st), hy = halo_y(st);
+  return hx < hpgx::halo_tile_limit() && hy < hpgx::halo_tile_limit()
+    && hpgx::halo_lds_bytes(hx, hy, 1) <= 65536;
+}
+
+// bin blocks per haloed-tile pass: the per-block segment tables share the
+// workgroup's 64 KiB of LDS with the tile
+size_t halo_bins_limit(hpgx_state_t st) {
+  size_t nb = size_t(hpgx::tile_bins_limit());
+  while (nb > 1 && hpgx::halo_lds_bytes(halo_x(st), halo_y(st), int(nb)) > 65536) nb /= 2;
+  return nb;
+}
+
 size_t entries_per_record(hpgx_state_t st) {
-  return size_t(std::max(1u, st->gsize[2]))
+  const size_t planes = size_t(std::max(1u, st->gsize[2]));
+  if (use_halo(st)) return planes;
+  return planes
     * hpgx::tiles_per_footprint(std::max(1, st->cf.max_sx), std::max(1, st->cf.max_sy));
 }
 
@@ -228,10 +262,47 @@ int ensure_call_buffers(hpgx_state_t st) {
     // bin entries for the whole batch, up to the entry memory budget; larger
     // batches are gridded in chunks (run_grid)
     const size_t epr = entries_per_record(st);
+    const bool halo = use_halo(st);
     const size_t per_entry = st->gridder == 1
-      ? sizeof(unsigned) : hpgx::entry_bytes(int(st->npol)) + sizeof(unsigned);
+      ? sizeof(unsigned)
+      : (halo ? hpgx::halo_entry_bytes(int(st->npol)) : hpgx::entry_bytes(int(st->npol)))
+        + sizeof(unsigned);
     const size_t budget = std::max(entry_budget_bytes() / per_entry, epr);
-    const size_t need = std::min(st->vis_capacity * epr, budget);
+    size_t need = std::min(st->vis_capacity * epr, budget);
+    if (use_block_local(st)) {
+      // whole bin blocks of bin_block_records records each
+      const size_t block_cap = hpgx::bin_block_records() * epr;
+      const size_t blocks_needed =
+        (st->vis_capacity + hpgx::bin_block_records() - 1) / hpgx::bin_block_records();
+      const size_t bins_max = halo ? halo_bins_limit(st) : size_t(hpgx::tile_bins_limit());
+      const size_t blocks =
+        std::max<size_t>(1, std::min({blocks_needed, budget / block_cap, bins_max}));
+      need = blocks * block_cap;
+      st->pass_blocks = blocks;
+      if (halo) {
+        const size_t hp =
+          st->nkeys * hpgx::halo_pixels_per_key(halo_x(st), halo_y(st));
+        if (hp > cb.halo_capacity) {
+          dev_free(cb.halo);
+          cb.halo_capacity = 0;
+          HPGX_HIP(st, dev_alloc(cb.halo, hp * sizeof(double2)));
+          cb.halo_capacity = hp;
+        }
+        if (st->nkeys > cb.halo_set_capacity) {
+          dev_free(cb.halo_set);
+          cb.halo_set_capacity = 0;
+          HPGX_HIP(st, dev_alloc(cb.halo_set, st->nkeys * sizeof(unsigned)));
+          cb.halo_set_capacity = st->nkeys;
+        }
+      }
+      const size_t tab = blocks * (st->nkeys + 1);
+      if (tab > cb.off_table_capacity) {
+        dev_free(cb.off_table);
+        cb.off_table_capacity = 0;
+        HPGX_HIP(st, dev_alloc(cb.off_table, tab * sizeof(unsigned)));
+        cb.off_table_capacity = tab;
+      }
+    }
     if (need > cb.entries_capacity) {
       dev_free(cb.entries);
       dev_free(cb.stream_entries);
@@ -240,7 +311,9 @@ int ensure_call_buffers(hpgx_state_t st) {
       if (st->gridder == 1) {
         HPGX_HIP(st, dev_alloc(cb.entries, need * sizeof(unsigned)));
       } else {
-        HPGX_HIP(st, dev_alloc(cb.stream_entries, need * hpgx::entry_bytes(int(st->npol))));
+        HPGX_HIP(st, dev_alloc(cb.stream_entries,
+                               need * std::max(hpgx::entry_bytes(int(st->npol)),
+                                               hpgx::halo_entry_bytes(int(st->npol)))));
         HPGX_HIP(st, dev_alloc(cb.entry_vis, need * sizeof(unsigned)));
       }
       cb.entries_capacity = need;
@@ -274,6 +347,9 @@ void free_state(hpgx_state_t st) {
   dev_free(cb.entries);
   dev_free(cb.stream_entries);
   dev_free(cb.entry_vis);
+  dev_free(cb.off_table);
+  dev_free(cb.halo);
+  dev_free(cb.halo_set);
   dev_free(cb.scan_temp);
   dev_free(cb.invalid_count);
   if (st->plan_z2z.valid) hipfftDestroy(st->plan_z2z.h);
@@ -452,6 +528,33 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
   if (!do_grid) return HPGX_SUCCESS;
   const int mode = st->mode == HPGX_GRID_FP64 ? 0 : 1;
   const size_t epr = entries_per_record(st);
+  if (use_block_local(st)) {
+    const bool halo = use_halo(st);
+    const size_t block_cap = hpgx::bin_block_records() * epr;
+    // bin blocks per pass as sized by ensure_call_buffers (entries, offset
+    // table and the tile kernel's LDS segment tables)
+    const size_t chunk = std::max<size_t>(1, st->pass_blocks) * hpgx::bin_block_records();
+    const size_t rec_bytes = visdata_size(st->npol);
+    for (size_t off = 0; off < n; off += chunk) {
+      const size_t m = std::min(chunk, n - off);
+      CallBuffers c = cb;
+      c.vis = static_cast<char*>(cb.vis) + off * rec_bytes;
+      c.gvis = cb.gvis + off * np;
+      const double2* res = residuals ? residuals + off * np : nullptr;
+      HPGX_HIP(st, hpgx::launch_bin(
+                     st->stream, np, m, geo, st->cf, st->mu, update_weights, gvis_ready, src,
+                     res, c, block_cap, st->weights, halo));
+      if (halo)
+        HPGX_HIP(st, hpgx::launch_tile_halo(
+                       st->stream, np, mode, m, st->nkeys, geo, st->cf, st->mu, c,
+                       block_cap, halo_x(st), halo_y(st), st->grid));
+      else
+        HPGX_HIP(st, hpgx::launch_tile_blocal(
+                       st->stream, np, mode, m, st->nkeys, geo, st->cf, st->mu, c,
+                       block_cap, st->grid));
+    }
+    return HPGX_SUCCESS;
+  }
   const size_t chunk = std::max<size_t>(1, cb.entries_capacity / epr);
   const size_t rec_bytes = visdata_size(st->npol);
   for (size_t off = 0; off < n; off += chunk) {
@@ -578,7 +681,10 @@ int hpgx_create(
   st->gscale[1] = grid_scale[1];
   st->npol = num_polarizations;
   st->mode = grid_mode;
-  if (const char* g = std::getenv("HPGX_GRIDDER")) st->gridder = std::atoi(g) == 1 ? 1 : 2;
+  if (const char* g = std::getenv("HPGX_GRIDDER")) {
+    const int v = std::atoi(g);
+    st->gridder = (v >= 1 && v <= 4) ? v : 3;
+  }
   // Mueller index matrices padded to 4x4 with -1
   // (copy_mueller_indexes, src/hpg_runtime.hpp:1590-1614)
   for (int r = 0; r < 4; ++r)

@@ -83,6 +83,12 @@ struct CallBuffers {
   size_t scan_temp_bytes;
   void* stream_entries;  // compact binned entries (Entry<N>), sorted by key
   unsigned* entry_vis;   // record index of entries with a phase gradient
+  unsigned* off_table;   // block-local bins: per bin block, nkeys + 1 offsets
+  size_t off_table_capacity;
+  double2* halo;         // haloed tiles: per key, the halo's FP64 values
+  unsigned* halo_set;    // per key: 1 if its halo was written by this pass
+  size_t halo_capacity;  // elements of halo
+  size_t halo_set_capacity;
   unsigned long long* invalid_count;
 };
 
@@ -130,6 +136,29 @@ hipError_t launch_emit(
 hipError_t launch_tile_stream(
   hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid);
+// block-local binning (small key counts)
+int bin_blocks_for(size_t n);
+size_t bin_block_records();
+int bin_keys_limit();
+int tile_bins_limit();
+hipError_t launch_bin(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, bool update_weights, bool gvis_ready, GvisSource src,
+  const double2* residuals, CallBuffers& cb, size_t block_cap, double* weights,
+  bool halo_entries);
+// haloed tiles (one entry per record and plane; supports below tile)
+size_t halo_entry_bytes(int npol);
+size_t halo_pixels_per_key(int hx, int hy);
+int halo_tile_limit();
+size_t halo_lds_bytes(int hx, int hy, int nbins);
+hipError_t launch_tile_halo(
+  hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
+  const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
+  int hx, int hy, void* grid);
+hipError_t launch_tile_blocal(
+  hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
+  const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
+  void* grid);
 hipError_t launch_degrid(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, const double2* model, bool do_grid, CallBuffers& cb,

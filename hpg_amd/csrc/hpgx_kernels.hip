@@ -837,6 +837,771 @@ __global__ __launch_bounds__(block) void k_tile2(
   }
 }
 
+// ---- gridding, block-local bins ----------------------------------------------
+//
+// For grids with few (plane, tile) keys (<= bin_keys_max) the bins are built
+// without any contended global atomic:
+// k_bin     one workgroup per 4096 records: a histogram of the block's
+//           entries over all keys in LDS, its exclusive scan, the per-block
+//           offset table off[block][key] (nkeys + 1 values), then each entry
+//           written at its block-local position -- one pass over the records.
+//           Weights are reduced per grid plane in LDS first.
+// k_tile3   one workgroup per key: gathers the key's segment from every bin
+//           block (offset table + LDS scan), then processes the virtual
+//           entry list 64 entries per wave; entries with at most 64 taps are
+//           handled four at a time with all their CF-tap loads issued
+//           together (latency overlap), the others tap-chunk by tap-chunk.
+
+constexpr int bin_threads = 1024;
+constexpr int bin_recs_per_thread = 4;
+constexpr int bin_records = bin_threads * bin_recs_per_thread;  // per block
+constexpr int bin_keys_max = 8192;
+constexpr int bin_planes_max = 1024;
+constexpr int tile_bins_max = 2048;  // bin blocks per pass (LDS of k_tile3)
+
+// block-wide exclusive scan of a[0, n) in LDS (n <= bin_keys_max), returns
+// the total; `part` holds blockDim.x / wave + 1 values
+__device__ unsigned block_exclusive_scan(unsigned* a, int n, unsigned* part) {
+  const int nt = blockDim.x;
+  const int per = (n + nt - 1) / nt;
+  const int beg = min(n, int(threadIdx.x) * per);
+  const int end = min(n, beg + per);
+  unsigned sum = 0;
+  for (int k = beg; k < end; ++k) sum += a[k];
+  // inclusive wave scan of the per-thread sums
+  const int lane = threadIdx.x % wave;
+  unsigned v = sum;
+#pragma unroll
+  for (int d = 1; d < wave; d <<= 1) {
+    const unsigned t = __shfl_up(v, d, wave);
+    if (lane >= d) v += t;
+  }
+  const int w = threadIdx.x / wave;
+  if (lane == wave - 1) part[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned acc = 0;
+    for (int k = 0; k < nt / wave; ++k) {
+      const unsigned t = part[k];
+      part[k] = acc;
+      acc += t;
+    }
+    part[nt / wave] = acc;
+  }
+  __syncthreads();
+  unsigned run = part[w] + v - sum;  // exclusive prefix of this thread
+  for (int k = beg; k < end; ++k) {
+    const unsigned t = a[k];
+    a[k] = run;
+    run += t;
+  }
+  const unsigned total = part[nt / wave];
+  __syncthreads();
+  return total;
+}
+
+// entries of the haloed-tile pipeline (k_tile4): one per (record, plane),
+// position relative to the tile that holds the support corner
+struct Entry4Head {
+  unsigned pos;  // lx | ly << 6 | fxi << 12 | fyi << 20 | flags << 28
+  unsigned gpk;  // grp | cfcube << 16
+};
+template <int N>
+struct Entry4 {
+  Entry4Head h;
+  float2 g[N];
+};
+constexpr unsigned e4_pos_w = 1u << 28;
+constexpr unsigned e4_gradient = 1u << 29;
+
+// HALO = false: one entry per (plane, tile) the support touches (Entry<N>,
+// for k_tile3); HALO = true: one entry per plane, keyed by the tile of the
+// support corner (Entry4<N>, for k_tile4)
+template <int N, bool HALO>
+__global__ __launch_bounds__(bin_threads) void k_bin(
+  const VisRec<N>* __restrict__ recs, size_t n, Geometry geo,
+  const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
+  Mueller mu, int update_weights, int gvis_ready, int adjoint,
+  const float2* __restrict__ gvis_in, const double2* __restrict__ residuals,
+  unsigned* __restrict__ off_table, void* __restrict__ entries_out,
+  unsigned* __restrict__ entry_vis, size_t block_cap, double* __restrict__ weights,
+  unsigned long long* __restrict__ invalid) {
+
+  __shared__ unsigned cnt[bin_keys_max];
+  __shared__ double wl[bin_planes_max];
+  __shared__ unsigned part[bin_threads / wave + 1];
+
+  const int nkeys = geo.ntx * geo.nty * geo.nr * geo.nc;
+  const int nplanes = geo.nr * geo.nc;
+  const bool local_w = nplanes <= bin_planes_max;
+  for (int k = threadIdx.x; k < nkeys; k += bin_threads) cnt[k] = 0;
+  if (local_w)
+    for (int k = threadIdx.x; k < nplanes; k += bin_threads) wl[k] = 0.0;
+  __syncthreads();
+
+  const int ntiles = geo.ntx * geo.nty;
+  const size_t b = blockIdx.x;
+  Entry<N> e[bin_recs_per_thread];
+  bool keep[bin_recs_per_thread];
+  unsigned short sxy[bin_recs_per_thread][2];
+  unsigned gc[bin_recs_per_thread];
+  unsigned ninvalid = 0;
+
+#pragma unroll
+  for (int j = 0; j < bin_recs_per_thread; ++j) {
+    keep[j] = false;
+    const size_t i = b * bin_records + size_t(j) * bin_threads + threadIdx.x;
+    if (i >= n) continue;
+    const VisRec<N>& r = recs[i];
+    const unsigned grp = r.cf_index[1];
+    const unsigned cfcube = r.cf_index[0];
+    const unsigned gcube = r.grid_cube;
+    bool valid = grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc);
+    if (valid) valid = cfcube < unsigned(groups[grp].ncube);
+    if (!valid) {
+      ++ninvalid;
+      continue;
+    }
+    const GroupInfo g = groups[grp];
+    const Coord c = vis_coord(r, geo, g);
+    if (!c.inside) continue;
+    const bool pos_w = r.uvw[2] > 0;
+    const bool grad = r.grad[0] != 0.0 || r.grad[1] != 0.0;
+    const int fxi = c.fx + geo.F, fyi = c.fy + geo.F;
+    keep[j] = true;
+    gc[j] = gcube;
+    sxy[j][0] = (unsigned short)g.sx;
+    sxy[j][1] = (unsigned short)g.sy;
+    e[j].h.gx = c.gx;
+    e[j].h.gy = c.gy;
+    e[j].h.fpk = unsigned(fxi) | (unsigned(fyi) << 8)
+      | (unsigned((pos_w ? flag_pos_w : 0) | (grad ? flag_gradient : 0)) << 16);
+    e[j].h.gpk = grp | (cfcube << 16);
+    if (gvis_ready) {
+      for (int v = 0; v < N; ++v) e[j].g[v] = gvis_in[i * N + v];
+    } else {
+      const double2 phasor = cphase(r.phase);
+      if (!adjoint) {
+        for (int v = 0; v < N; ++v) e[j].g[v] = prephase(r.vis[v], phasor, r.w[v]);
+      } else {
+        double2 W[N];
+        degrid_norms<N>(geo, g, wsum, mu, pos_w, cfcube, fxi, fyi, W);
+        for (int v = 0; v < N; ++v) {
+          double2 d = W[v];
+          if (d.x == 0.0 && d.y == 0.0) d = make_double2(1.0, 0.0);
+          d.y = -d.y;
+          e[j].g[v] = narrow(kdiv(cmul(residuals[i * N + v], phasor), d));
+        }
+      }
+    }
+    if (update_weights) {
+      // twgt = sum_v |sum_XY cf'| * w_v (src/hpg_core.hpp:877-885)
+      for (int gpol = 0; gpol < geo.nr; ++gpol) {
+        const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
+        double t = 0.0;
+        for (int v = 0; v < N; ++v) {
+          const int m = row[v];
+          double mag = 0.0;
+          if (m >= 0 && m < g.nmueller) {
+            const double2 sw = wsum[g.wsum_offset
+                                    + ((long long)(cfcube * g.nmueller + m) * geo.nf + fxi)
+                                      * geo.nf + fyi];
+            mag = hypot(sw.x, sw.y);
+          }
+          t += mag * double(r.w[v]);
+        }
+        double* dst = local_w ? wl + gcube * geo.nr + gpol
+                              : weights + size_t(gcube) * geo.nr + gpol;
+        if (local_w)
+          __hip_atomic_fetch_add(dst, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+          atomicAdd(dst, t);
+      }
+    }
+    const int tx0 = tiles_begin(c.gx), tx1 = HALO ? tx0 + 1 : tiles_end(c.gx, g.sx);
+    const int ty0 = tiles_begin(c.gy), ty1 = HALO ? ty0 + 1 : tiles_end(c.gy, g.sy);
+    for (int gpol = 0; gpol < geo.nr; ++gpol) {
+      if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
+      const int base = (int(gcube) * geo.nr + gpol) * ntiles;
+      for (int ty = ty0; ty < ty1; ++ty)
+        for (int tx = tx0; tx < tx1; ++tx)
+          __hip_atomic_fetch_add(cnt + base + ty * geo.ntx + tx, 1u, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  // invalid records: one global atomic per wave at most
+  {
+    unsigned long long inv = ninvalid;
+#pragma unroll
+    for (int m = wave / 2; m > 0; m >>= 1) inv += __shfl_xor(inv, m, wave);
+    if (threadIdx.x % wave == 0 && inv) atomicAdd(invalid, inv);
+  }
+  __syncthreads();
+  block_exclusive_scan(cnt, nkeys, part);
+  unsigned* tab = off_table + b * size_t(nkeys + 1);
+  for (int k = threadIdx.x; k < nkeys; k += bin_threads) tab[k] = cnt[k];
+  if (threadIdx.x == 0) tab[nkeys] = part[bin_threads / wave];
+  if (update_weights && local_w)
+    for (int k = threadIdx.x; k < nplanes; k += bin_threads)
+      if (wl[k] != 0.0) atomicAdd(weights + k, wl[k]);
+  __syncthreads();
+
+  unsigned* outv = entry_vis + b * block_cap;
+#pragma unroll
+  for (int j = 0; j < bin_recs_per_thread; ++j) {
+    if (!keep[j]) continue;
+    const size_t i = size_t(j) * bin_threads + threadIdx.x;
+    const bool pos_w = (e[j].h.fpk >> 16) & flag_pos_w;
+    const bool grad = (e[j].h.fpk >> 16) & flag_gradient;
+    const int tx0 = tiles_begin(e[j].h.gx);
+    const int tx1 = HALO ? tx0 + 1 : tiles_end(e[j].h.gx, sxy[j][0]);
+    const int ty0 = tiles_begin(e[j].h.gy);
+    const int ty1 = HALO ? ty0 + 1 : tiles_end(e[j].h.gy, sxy[j][1]);
+    for (int gpol = 0; gpol < geo.nr; ++gpol) {
+      if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
+      const int base = (int(gc[j]) * geo.nr + gpol) * ntiles;
+      for (int ty = ty0; ty < ty1; ++ty)
+        for (int tx = tx0; tx < tx1; ++tx) {
+          const unsigned pos = __hip_atomic_fetch_add(
+            cnt + base + ty * geo.ntx + tx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if constexpr (HALO) {
+            Entry4<N> q;
+            q.h.pos = unsigned(e[j].h.gx - tx0 * tile) | (unsigned(e[j].h.gy - ty0 * tile) << 6)
+              | ((e[j].h.fpk & 0xffffu) << 12) | (pos_w ? e4_pos_w : 0u)
+              | (grad ? e4_gradient : 0u);
+            q.h.gpk = e[j].h.gpk;
+            for (int v = 0; v < N; ++v) q.g[v] = e[j].g[v];
+            static_cast<Entry4<N>*>(entries_out)[b * block_cap + pos] = q;
+          } else {
+            static_cast<Entry<N>*>(entries_out)[b * block_cap + pos] = e[j];
+          }
+          if (grad) outv[pos] = unsigned(b * bin_records + i);
+        }
+    }
+  }
+}
+
+// parameters of one entry, uniform across the wave
+struct EntryParams {
+  int gx, gy, fxi, fyi, grp, cfcube, sx;
+  unsigned flags;
+};
+
+template <int N, int MODE, int U>
+__global__ __launch_bounds__(block) void k_tile3(
+  Geometry geo, const GroupInfo* __restrict__ groups,
+  const float2* __restrict__ taps, Mueller mu, const unsigned* __restrict__ off_table,
+  int nbins, size_t block_cap, const Entry<N>* __restrict__ entries,
+  const unsigned* __restrict__ entry_vis, const VisRec<N>* __restrict__ recs,
+  void* __restrict__ grid) {
+
+  constexpr int TT = tile * tile;
+  __shared__ double acc64[MODE == 0 ? 2 * TT : 1];
+  __shared__ float acc32[MODE == 1 ? 4 * TT : 1];
+  __shared__ unsigned seg_pref[tile_bins_max + 1];
+  __shared__ unsigned seg_beg[tile_bins_max];
+  __shared__ unsigned part[block / wave + 1];
+
+  const unsigned key = blockIdx.x;
+  const int nkeys = geo.ntx * geo.nty * geo.nr * geo.nc;
+  // segment of this key in every bin block
+  for (int bb = threadIdx.x; bb < nbins; bb += block) {
+    const unsigned* tab = off_table + size_t(bb) * (nkeys + 1);
+    const unsigned s0 = tab[key], s1 = tab[key + 1];
+    seg_pref[bb] = s1 - s0;
+    seg_beg[bb] = s0;
+  }
+  __syncthreads();
+  const unsigned total = block_exclusive_scan(seg_pref, nbins, part);
+  if (total == 0) return;
+  if (threadIdx.x == 0) seg_pref[nbins] = total;
+
+  const int ntiles = geo.ntx * geo.nty;
+  const unsigned plane = key / ntiles;
+  const int t = int(key % ntiles);
+  const int x0 = (t % geo.ntx) * tile;
+  const int y0 = (t / geo.ntx) * tile;
+  const int gpol = int(plane % geo.nr);
+
+  if constexpr (MODE == 0) {
+    for (int k = threadIdx.x; k < 2 * TT; k += block) acc64[k] = 0.0;
+  } else {
+    for (int k = threadIdx.x; k < 4 * TT; k += block) acc32[k] = 0.0f;
+  }
+  __syncthreads();
+
+  const int w = threadIdx.x / wave;
+  const int lane = threadIdx.x % wave;
+
+  auto accumulate = [&](int li, double sr, double si) {
+    if constexpr (MODE == 0) {
+      __hip_atomic_fetch_add(acc64 + li, sr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(acc64 + TT + li, si, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+      const float hr = float(sr), hi = float(si);
+      const float lr = float(sr - double(hr)), lim = float(si - double(hi));
+      const float orr = __hip_atomic_fetch_add(acc32 + li, hr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const float ori = __hip_atomic_fetch_add(acc32 + TT + li, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      float s, er, ei;
+      two_sum(orr, hr, s, er);
+      two_sum(ori, hi, s, ei);
+      __hip_atomic_fetch_add(acc32 + 2 * TT + li, er + lr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(acc32 + 3 * TT + li, ei + lim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  };
+
+  for (unsigned base = w * wave; base < total; base += tile_waves * wave) {
+    const int cnt = int(min(unsigned(wave), total - base));
+    Entry<N> mine;
+    unsigned mine_idx = 0;
+    if (lane < cnt) {
+      // locate entry base + lane in the concatenated segments
+      const unsigned q = base + lane;
+      int lo = 0, hi = nbins;  // seg_pref[lo] <= q < seg_pref[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (seg_pref[mid] <= q) lo = mid; else hi = mid;
+      }
+      mine_idx = unsigned(lo) * unsigned(block_cap) + seg_beg[lo] + (q - seg_pref[lo]);
+      mine = entries[mine_idx];
+    }
+    for (int k0 = 0; k0 < cnt; k0 += U) {
+      EntryParams P[U];
+      float2 gv[U][N];
+      int xa[U], wdt[U], ya[U], ntap[U];
+      const float2* slice[U][N];
+      bool use[U][N];
+      bool fast = true;
+      for (int u = 0; u < U; ++u) {
+        const int k = min(k0 + u, cnt - 1);
+        P[u].gx = bcast(mine.h.gx, k);
+        P[u].gy = bcast(mine.h.gy, k);
+        const unsigned fpk = unsigned(bcast(int(mine.h.fpk), k));
+        const unsigned gpk = unsigned(bcast(int(mine.h.gpk), k));
+        P[u].fxi = int(fpk & 0xffu);
+        P[u].fyi = int((fpk >> 8) & 0xffu);
+        P[u].flags = fpk >> 16;
+        P[u].grp = int(gpk & 0xffffu);
+        P[u].cfcube = int(gpk >> 16);
+        for (int v = 0; v < N; ++v)
+          gv[u][v] = make_float2(bcast(mine.g[v].x, k), bcast(mine.g[v].y, k));
+        const GroupInfo g = groups[P[u].grp];
+        xa[u] = max(P[u].gx, x0) - P[u].gx;
+        const int xb = min(P[u].gx + g.sx, x0 + tile) - P[u].gx;
+        ya[u] = max(P[u].gy, y0) - P[u].gy;
+        const int yb = min(P[u].gy + g.sy, y0 + tile) - P[u].gy;
+        wdt[u] = xb - xa[u];
+        ntap[u] = k0 + u < cnt ? wdt[u] * (yb - ya[u]) : 0;
+        const bool pos_w = P[u].flags & flag_pos_w;
+        const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
+        const long long slice_taps = (long long)g.sx * g.sy;
+        for (int v = 0; v < N; ++v) {
+          const int m = row[v];
+          use[u][v] = m >= 0 && m < g.nmueller;
+          slice[u][v] = taps + g.tap_offset
+            + (((long long)(P[u].cfcube * g.nmueller + (use[u][v] ? m : 0)) * geo.nf + P[u].fxi)
+                 * geo.nf + P[u].fyi) * slice_taps;
+        }
+        fast = fast && ntap[u] <= wave && !(P[u].flags & flag_gradient);
+        P[u].sx = g.sx;
+      }
+      if (fast) {
+        // all loads first, then the products and the LDS accumulation
+        float2 c[U][N];
+        int li[U];
+        for (int u = 0; u < U; ++u) {
+          const unsigned mg = magic(unsigned(max(wdt[u], 1)));
+          const int yl = int(div_magic(unsigned(lane), unsigned(max(wdt[u], 1)), mg));
+          const int X = xa[u] + (lane - yl * wdt[u]);
+          const int Y = ya[u] + yl;
+          li[u] = (P[u].gy + Y - y0) * tile + (P[u].gx + X - x0);
+          for (int v = 0; v < N; ++v)
+            c[u][v] = (lane < ntap[u] && use[u][v]) ? slice[u][v][Y * P[u].sx + X]
+                                                    : make_float2(0.0f, 0.0f);
+        }
+        for (int u = 0; u < U; ++u) {
+          if (lane >= ntap[u]) continue;
+          const float imf = (P[u].flags & flag_pos_w) ? -1.0f : 1.0f;
+          double sr = 0.0, si = 0.0;
+          for (int v = 0; v < N; ++v) {
+            if (!use[u][v]) continue;
+            float2 cc = c[u][v];
+            cc.y *= imf;
+            const float2 pr = cmul(cc, gv[u][v]);
+            sr += double(pr.x);
+            si += double(pr.y);
+          }
+          accumulate(li[u], sr, si);
+        }
+      } else {
+        for (int u = 0; u < U; ++u) {
+          if (ntap[u] == 0) continue;
+          const int k = k0 + u;
+          const bool grad = P[u].flags & flag_gradient;
+          const float imf = (P[u].flags & flag_pos_w) ? -1.0f : 1.0f;
+          const int sx = P[u].sx;
+          double phi0x = 0.0, phi0y = 0.0, dphix = 0.0, dphiy = 0.0;
+          if (grad) {
+            const unsigned idx = unsigned(bcast(int(mine_idx), k));
+            const unsigned i = entry_vis[idx];
+            const GroupInfo g = groups[P[u].grp];
+            const double gxr = recs[i].grad[0];
+            const double gyr = recs[i].grad[1];
+            phi0x = -gxr * ((g.sx / 2) * geo.os - (P[u].fxi - geo.F));
+            phi0y = -gyr * ((g.sy / 2) * geo.os - (P[u].fyi - geo.F));
+            dphix = gxr * geo.os;
+            dphiy = gyr * geo.os;
+          }
+          const unsigned mg = magic(unsigned(wdt[u]));
+          for (int q = lane; q < ntap[u]; q += wave) {
+            const int yl = int(div_magic(unsigned(q), unsigned(wdt[u]), mg));
+            const int X = xa[u] + (q - yl * wdt[u]);
+            const int Y = ya[u] + yl;
+            float2 scr = make_float2(1.0f, 0.0f);
+            if (grad) {
+              const double phX = phi0x + X * dphix;
+              const double phY = phi0y + Y * dphiy;
+              scr = narrow(cphase(phX + phY));
+            }
+            double sr = 0.0, si = 0.0;
+            for (int v = 0; v < N; ++v) {
+              if (!use[u][v]) continue;
+              float2 cc = slice[u][v][Y * sx + X];
+              cc.y *= imf;
+              if (grad) cc = cmul(cc, scr);
+              const float2 pr = cmul(cc, gv[u][v]);
+              sr += double(pr.x);
+              si += double(pr.y);
+            }
+            accumulate((P[u].gy + Y - y0) * tile + (P[u].gx + X - x0), sr, si);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int k = threadIdx.x; k < TT; k += block) {
+    const int xx = x0 + k % tile;
+    const int yy = y0 + k / tile;
+    if (xx >= geo.nx || yy >= geo.ny) continue;
+    const size_t gi = (size_t(plane) * geo.ny + yy) * geo.nx + xx;
+    if constexpr (MODE == 0) {
+      double2* gp = static_cast<double2*>(grid) + gi;
+      double2 gvl = *gp;
+      gvl.x += acc64[k];
+      gvl.y += acc64[TT + k];
+      *gp = gvl;
+    } else {
+      float2* gp = static_cast<float2*>(grid) + gi;
+      const float2 g0 = *gp;
+      float sr, er, si, ei;
+      two_sum(g0.x, acc32[k], sr, er);
+      two_sum(g0.y, acc32[TT + k], si, ei);
+      *gp = make_float2(sr + (er + acc32[2 * TT + k]), si + (ei + acc32[3 * TT + k]));
+    }
+  }
+}
+
+// ---- gridding, haloed tiles ---------------------------------------------
+//
+// k_bin<N, true> writes one Entry4 per (record, plane) into the bin of the
+// tile that holds the support corner.  The workgroup of a key accumulates
+// the tile plus a halo of hx = max_sx - 1 columns and hy = max_sy - 1 rows on
+// its high sides (R = (tile + hx) x (tile + hy) pixels in LDS), so every tap
+// of an entry lands in LDS without clipping.  One lane carries one entry
+// through all of its taps: per tap a CF load, the FP32 product and the LDS
+// accumulation, with no cross-lane traffic.  The workgroup then adds its own
+// tile into the grid (it is the only writer of those pixels during the
+// kernel) and stores the halo, as FP64 values, into a per-key side buffer;
+// k_halo adds each tile's neighbours' halos into it afterwards, in a fixed
+// order.  Requires hx, hy < tile.
+
+constexpr int halo_block = 256;
+constexpr int halo_chunk = 8;  // CF taps loaded together per lane
+
+// pixels of the halo of one key: columns [tile, tile + hx) of rows
+// [0, tile), then all columns of rows [tile, tile + hy)
+__host__ __device__ inline int halo_pixels(int hx, int hy) {
+  return hx * tile + (tile + hx) * hy;
+}
+
+template <int N, int MODE>
+__global__ __launch_bounds__(halo_block) void k_tile4(
+  Geometry geo, const GroupInfo* __restrict__ groups,
+  const float2* __restrict__ taps, Mueller mu, const unsigned* __restrict__ off_table,
+  int nbins, size_t block_cap, const Entry4<N>* __restrict__ entries,
+  const unsigned* __restrict__ entry_vis, const VisRec<N>* __restrict__ recs,
+  int hx, int hy, void* __restrict__ grid, double2* __restrict__ halo,
+  unsigned* __restrict__ halo_set) {
+
+  extern __shared__ double smem[];
+  const int RX = tile + hx;
+  const int RY = tile + hy;
+  const int R = RX * RY;
+  // LDS: accumulators (16 bytes per region pixel), then the segment tables
+  double* acc64 = smem;                                // MODE 0: re[R], im[R]
+  float* acc32 = reinterpret_cast<float*>(smem);       // MODE 1: hr, hi, lr, li
+  unsigned* seg_pref = reinterpret_cast<unsigned*>(smem + 2 * R);
+  unsigned* seg_beg = seg_pref + nbins + 1;
+  unsigned* part = seg_beg + nbins;
+
+  const unsigned key = blockIdx.x;
+  const int nkeys = geo.ntx * geo.nty * geo.nr * geo.nc;
+  for (int bb = threadIdx.x; bb < nbins; bb += halo_block) {
+    const unsigned* tab = off_table + size_t(bb) * (nkeys + 1);
+    const unsigned s0 = tab[key], s1 = tab[key + 1];
+    seg_pref[bb] = s1 - s0;
+    seg_beg[bb] = s0;
+  }
+  __syncthreads();
+  const unsigned total = block_exclusive_scan(seg_pref, nbins, part);
+  if (total == 0) {
+    if (threadIdx.x == 0) halo_set[key] = 0u;
+    return;
+  }
+  if (threadIdx.x == 0) seg_pref[nbins] = total;
+
+  const int ntiles = geo.ntx * geo.nty;
+  const unsigned plane = key / ntiles;
+  const int t = int(key % ntiles);
+  const int x0 = (t % geo.ntx) * tile;
+  const int y0 = (t / geo.ntx) * tile;
+  const int gpol = int(plane % geo.nr);
+
+  for (int k = threadIdx.x; k < 2 * R; k += halo_block) acc64[k] = 0.0;
+  __syncthreads();
+
+  const int w = threadIdx.x / wave;
+  const int lane = threadIdx.x % wave;
+  constexpr int nw = halo_block / wave;
+
+  for (unsigned base = w * wave; base < total; base += nw * wave) {
+    const unsigned q = base + lane;
+    if (q >= total) continue;
+    // locate entry q in the concatenated segments
+    int lo = 0, hi = nbins;  // seg_pref[lo] <= q < seg_pref[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (seg_pref[mid] <= q) lo = mid; else hi = mid;
+    }
+    const unsigned idx = unsigned(lo) * unsigned(block_cap) + seg_beg[lo] + (q - seg_pref[lo]);
+    const Entry4<N> e = entries[idx];
+    const unsigned pos = e.h.pos;
+    const int lx = int(pos & 63u);
+    const int ly = int((pos >> 6) & 63u);
+    const int fxi = int((pos >> 12) & 255u);
+    const int fyi = int((pos >> 20) & 255u);
+    const bool pos_w = pos & e4_pos_w;
+    const bool grad = pos & e4_gradient;
+    const int grp = int(e.h.gpk & 0xffffu);
+    const int cfcube = int(e.h.gpk >> 16);
+    const GroupInfo g = groups[grp];
+    const int sx = g.sx, sy = g.sy;
+    const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
+    const float imf = pos_w ? -1.0f : 1.0f;
+    const float2* slice[N];
+    bool use[N];
+    const long long slice_taps = (long long)sx * sy;
+    for (int v = 0; v < N; ++v) {
+      const int m = row[v];
+      use[v] = m >= 0 && m < g.nmueller;
+      slice[v] = taps + g.tap_offset
+        + (((long long)(cfcube * g.nmueller + (use[v] ? m : 0)) * geo.nf + fxi) * geo.nf
+           + fyi) * slice_taps;
+    }
+    double phi0x = 0.0, phi0y = 0.0, dphix = 0.0, dphiy = 0.0;
+    if (grad) {
+      const unsigned i = entry_vis[idx];
+      const double gxr = recs[i].grad[0];
+      const double gyr = recs[i].grad[1];
+      phi0x = -gxr * ((sx / 2) * geo.os - (fxi - geo.F));
+      phi0y = -gyr * ((sy / 2) * geo.os - (fyi - geo.F));
+      dphix = gxr * geo.os;
+      dphiy = gyr * geo.os;
+    }
+    for (int Y = 0; Y < sy; ++Y) {
+      const int lrow = (ly + Y) * RX + lx;
+      for (int X0 = 0; X0 < sx; X0 += halo_chunk) {
+        float2 c[N][halo_chunk];
+#pragma unroll
+        for (int k = 0; k < halo_chunk; ++k)
+#pragma unroll
+          for (int v = 0; v < N; ++v)
+            c[v][k] = (X0 + k < sx && use[v]) ? slice[v][Y * sx + X0 + k]
+                                              : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int k = 0; k < halo_chunk; ++k) {
+          if (X0 + k >= sx) break;
+          const int X = X0 + k;
+          float2 scr = make_float2(1.0f, 0.0f);
+          if (grad) scr = narrow(cphase((phi0x + X * dphix) + (phi0y + Y * dphiy)));
+          const int li = lrow + X;
+          if constexpr (N == 1) {
+            float2 cc = c[0][k];
+            cc.y *= imf;
+            if (grad) cc = cmul(cc, scr);
+            const float2 pr = use[0] ? cmul(cc, e.g[0]) : make_float2(0.0f, 0.0f);
+            if constexpr (MODE == 0) {
+              __hip_atomic_fetch_add(acc64 + li, double(pr.x), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+              __hip_atomic_fetch_add(acc64 + R + li, double(pr.y), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+              const float orr = __hip_atomic_fetch_add(
+                acc32 + li, pr.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              const float ori = __hip_atomic_fetch_add(
+                acc32 + R + li, pr.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              float s, er, ei;
+              two_sum(orr, pr.x, s, er);
+              two_sum(ori, pr.y, s, ei);
+              __hip_atomic_fetch_add(acc32 + 2 * R + li, er, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+              __hip_atomic_fetch_add(acc32 + 3 * R + li, ei, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+          } else {
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int v = 0; v < N; ++v) {
+              if (!use[v]) continue;
+              float2 cc = c[v][k];
+              cc.y *= imf;
+              if (grad) cc = cmul(cc, scr);
+              const float2 pr = cmul(cc, e.g[v]);
+              sr += double(pr.x);
+              si += double(pr.y);
+            }
+            if constexpr (MODE == 0) {
+              __hip_atomic_fetch_add(acc64 + li, sr, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+              __hip_atomic_fetch_add(acc64 + R + li, si, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+              const float hr = float(sr), him = float(si);
+              const float lr = float(sr - double(hr)), lim = float(si - double(him));
+              const float orr = __hip_atomic_fetch_add(
+                acc32 + li, hr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              const float ori = __hip_atomic_fetch_add(
+                acc32 + R + li, him, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              float s, er, ei;
+              two_sum(orr, hr, s, er);
+              two_sum(ori, him, s, ei);
+              __hip_atomic_fetch_add(acc32 + 2 * R + li, er + lr, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+              __hip_atomic_fetch_add(acc32 + 3 * R + li, ei + lim, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  const int hs = halo_pixels(hx, hy);
+  double2* hk = halo + size_t(key) * hs;
+  for (int k = threadIdx.x; k < R; k += halo_block) {
+    const int ry = k / RX;
+    const int rx = k - ry * RX;
+    double vr, vi;
+    float fr = 0.0f, fi = 0.0f, lr = 0.0f, li = 0.0f;
+    if constexpr (MODE == 0) {
+      vr = acc64[k];
+      vi = acc64[R + k];
+    } else {
+      fr = acc32[k];
+      fi = acc32[R + k];
+      lr = acc32[2 * R + k];
+      li = acc32[3 * R + k];
+      vr = double(fr) + double(lr);
+      vi = double(fi) + double(li);
+    }
+    if (rx < tile && ry < tile) {
+      const int xx = x0 + rx, yy = y0 + ry;
+      if (xx >= geo.nx || yy >= geo.ny) continue;
+      const size_t gi = (size_t(plane) * geo.ny + yy) * geo.nx + xx;
+      if constexpr (MODE == 0) {
+        double2* gp = static_cast<double2*>(grid) + gi;
+        double2 gvl = *gp;
+        gvl.x += vr;
+        gvl.y += vi;
+        *gp = gvl;
+      } else {
+        float2* gp = static_cast<float2*>(grid) + gi;
+        const float2 g0 = *gp;
+        float sr, er, si, ei;
+        two_sum(g0.x, fr, sr, er);
+        two_sum(g0.y, fi, si, ei);
+        *gp = make_float2(sr + (er + lr), si + (ei + li));
+      }
+    } else {
+      const int h = ry < tile ? ry * hx + (rx - tile) : tile * hx + (ry - tile) * RX + rx;
+      hk[h] = make_double2(vr, vi);
+    }
+  }
+  if (threadIdx.x == 0) halo_set[key] = 1u;
+}
+
+// adds the halos of each tile's left, upper and upper-left neighbours into
+// the tile (pixels (u, v) of the tile with u < hx or v < hy), one thread per
+// pixel, in that fixed order
+template <typename T>
+__global__ void k_halo(
+  Geometry geo, int hx, int hy, const double2* __restrict__ halo,
+  const unsigned* __restrict__ halo_set, size_t nkeys, T* __restrict__ grid) {
+  const int zone = hy * tile + (tile - hy) * hx;
+  const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (zone == 0 || tid >= nkeys * zone) return;
+  const size_t key = tid / zone;
+  const int z = int(tid - key * zone);
+  int u, v;
+  if (z < hy * tile) {
+    v = z / tile;
+    u = z - v * tile;
+  } else {
+    const int z2 = z - hy * tile;
+    v = hy + z2 / hx;
+    u = z2 - (v - hy) * hx;
+  }
+  const int ntiles = geo.ntx * geo.nty;
+  const size_t plane = key / ntiles;
+  const int t = int(key - plane * ntiles);
+  const int TX = t % geo.ntx;
+  const int TY = t / geo.ntx;
+  const int x = TX * tile + u;
+  const int y = TY * tile + v;
+  if (x >= geo.nx || y >= geo.ny) return;
+  const int hs = halo_pixels(hx, hy);
+  const int RX = tile + hx;
+  double sr = 0.0, si = 0.0;
+  bool any = false;
+  if (TX > 0 && u < hx && halo_set[key - 1]) {
+    const double2 h = halo[(key - 1) * hs + v * hx + u];
+    sr += h.x;
+    si += h.y;
+    any = true;
+  }
+  if (TY > 0 && v < hy && halo_set[key - geo.ntx]) {
+    const double2 h = halo[(key - geo.ntx) * hs + tile * hx + v * RX + u];
+    sr += h.x;
+    si += h.y;
+    any = true;
+  }
+  if (TX > 0 && TY > 0 && u < hx && v < hy && halo_set[key - geo.ntx - 1]) {
+    const double2 h = halo[(key - geo.ntx - 1) * hs + tile * hx + v * RX + tile + u];
+    sr += h.x;
+    si += h.y;
+    any = true;
+  }
+  if (!any) return;
+  T* gp = grid + (plane * geo.ny + y) * geo.nx + x;
+  T g = *gp;
+  g.x = double(g.x) + sr;
+  g.y = double(g.y) + si;
+  *gp = g;
+}
+
 // ---- degridding --------------------------------------------------------
 
 template <int N>
@@ -1211,6 +1976,139 @@ hipError_t launch_tile_stream(
     }
   }
 #undef HPGX_TILE2
+  return hipGetLastError();
+}
+
+int bin_blocks_for(size_t n) { return int((n + bin_records - 1) / bin_records); }
+size_t bin_block_records() { return bin_records; }
+int bin_keys_limit() { return bin_keys_max; }
+int tile_bins_limit() { return tile_bins_max; }
+
+hipError_t launch_bin(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, bool update_weights, bool gvis_ready, GvisSource src,
+  const double2* residuals, CallBuffers& cb, size_t block_cap, double* weights,
+  bool halo_entries) {
+  if (n == 0) return hipSuccess;
+  const int adj = src == GvisSource::Adjoint;
+  const unsigned nb = unsigned(bin_blocks_for(n));
+#define HPGX_BIN(NP, H)                                                        \
+  hipLaunchKernelGGL(                                                          \
+    (k_bin<NP, H>), dim3(nb), dim3(bin_threads), 0, s,                         \
+    static_cast<const VisRec<NP>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,    \
+    int(update_weights), int(gvis_ready), adj, cb.gvis, residuals,             \
+    cb.off_table, cb.stream_entries, cb.entry_vis, block_cap, weights,         \
+    cb.invalid_count)
+  if (halo_entries) {
+    switch (npol) {
+    case 1: HPGX_BIN(1, true); break;
+    case 2: HPGX_BIN(2, true); break;
+    case 3: HPGX_BIN(3, true); break;
+    case 4: HPGX_BIN(4, true); break;
+    default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (npol) {
+    case 1: HPGX_BIN(1, false); break;
+    case 2: HPGX_BIN(2, false); break;
+    case 3: HPGX_BIN(3, false); break;
+    case 4: HPGX_BIN(4, false); break;
+    default: return hipErrorInvalidValue;
+    }
+  }
+#undef HPGX_BIN
+  return hipGetLastError();
+}
+
+size_t halo_entry_bytes(int npol) {
+  return sizeof(Entry4Head) + sizeof(float2) * size_t(npol);
+}
+size_t halo_pixels_per_key(int hx, int hy) { return size_t(halo_pixels(hx, hy)); }
+int halo_tile_limit() { return tile; }
+size_t halo_lds_bytes(int hx, int hy, int nbins) {
+  const size_t r = size_t(tile + hx) * (tile + hy);
+  return r * 16 + sizeof(unsigned) * (2 * size_t(nbins) + 1 + halo_block / wave + 1);
+}
+
+hipError_t launch_tile_halo(
+  hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
+  const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
+  int hx, int hy, void* grid) {
+  if (nkeys == 0 || n == 0) return hipSuccess;
+  if (hx < 0 || hy < 0 || hx >= tile || hy >= tile) return hipErrorInvalidValue;
+  const int nb = bin_blocks_for(n);
+  const size_t lds = halo_lds_bytes(hx, hy, nb);
+  if (lds > 65536) return hipErrorInvalidValue;
+#define HPGX_TILE4(NP, MD)                                                     \
+  hipLaunchKernelGGL(                                                          \
+    (k_tile4<NP, MD>), dim3(unsigned(nkeys)), dim3(halo_block), lds, s, geo,   \
+    cf.groups, cf.taps, mu, cb.off_table, nb, block_cap,                       \
+    static_cast<const Entry4<NP>*>(cb.stream_entries), cb.entry_vis,           \
+    static_cast<const VisRec<NP>*>(cb.vis), hx, hy, grid, cb.halo, cb.halo_set)
+  if (mode == 0) {
+    switch (npol) {
+    case 1: HPGX_TILE4(1, 0); break;
+    case 2: HPGX_TILE4(2, 0); break;
+    case 3: HPGX_TILE4(3, 0); break;
+    case 4: HPGX_TILE4(4, 0); break;
+    default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (npol) {
+    case 1: HPGX_TILE4(1, 1); break;
+    case 2: HPGX_TILE4(2, 1); break;
+    case 3: HPGX_TILE4(3, 1); break;
+    case 4: HPGX_TILE4(4, 1); break;
+    default: return hipErrorInvalidValue;
+    }
+  }
+#undef HPGX_TILE4
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const size_t zone = size_t(hy) * tile + size_t(tile - hy) * hx;
+  const size_t count = nkeys * zone;
+  if (count == 0) return hipSuccess;
+  if (mode == 0)
+    hipLaunchKernelGGL(
+      k_halo<double2>, dim3(blocks_for(count)), dim3(block), 0, s, geo, hx, hy,
+      cb.halo, cb.halo_set, nkeys, static_cast<double2*>(grid));
+  else
+    hipLaunchKernelGGL(
+      k_halo<float2>, dim3(blocks_for(count)), dim3(block), 0, s, geo, hx, hy,
+      cb.halo, cb.halo_set, nkeys, static_cast<float2*>(grid));
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_blocal(
+  hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
+  const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
+  void* grid) {
+  if (nkeys == 0 || n == 0) return hipSuccess;
+  const int nb = bin_blocks_for(n);
+#define HPGX_TILE3(NP, MD, UU)                                                 \
+  hipLaunchKernelGGL(                                                          \
+    (k_tile3<NP, MD, UU>), dim3(unsigned(nkeys)), dim3(block), 0, s, geo,      \
+    cf.groups, cf.taps, mu, cb.off_table, nb, block_cap,                       \
+    static_cast<const Entry<NP>*>(cb.stream_entries), cb.entry_vis,            \
+    static_cast<const VisRec<NP>*>(cb.vis), grid)
+  if (mode == 0) {
+    switch (npol) {
+    case 1: HPGX_TILE3(1, 0, 4); break;
+    case 2: HPGX_TILE3(2, 0, 4); break;
+    case 3: HPGX_TILE3(3, 0, 2); break;
+    case 4: HPGX_TILE3(4, 0, 2); break;
+    default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (npol) {
+    case 1: HPGX_TILE3(1, 1, 4); break;
+    case 2: HPGX_TILE3(2, 1, 4); break;
+    case 3: HPGX_TILE3(3, 1, 2); break;
+    case 4: HPGX_TILE3(4, 1, 2); break;
+    default: return hipErrorInvalidValue;
+    }
+  }
+#undef HPGX_TILE3
   return hipGetLastError();
 }
 
