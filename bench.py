@@ -199,7 +199,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "grid_visibilities call (k_bin + k_tile3)",
+                     "kernel": "grid_visibilities call (k_bin + k_tile4 + k_halo)",
                      "bytes_per_call": nbytes, "device_ms_per_call": dev_ms,
                      "fp32_frac": flops / (dev_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS},
         "finishing_ms": fin,

@@ -196,6 +196,10 @@ def _as_records(vis, npol):
             nbytes = vis.numel() * vis.element_size()
             if not vis.is_contiguous() or nbytes % itemsize:
                 raise ValueError("device visibilities must be contiguous VisData records")
+            # the state's stream reads the records in place: work queued on
+            # torch's stream that writes them must be done first
+            import torch
+            torch.cuda.current_stream(vis.device).synchronize()
             return ctypes.c_void_p(vis.data_ptr()), nbytes // itemsize, 1, vis
     arr = np.asarray(vis)
     dt = visdata_dtype(npol)
@@ -215,6 +219,8 @@ class GridderState:
 
     def __init__(self, handle):
         self._h = handle
+        # device arrays read by queued work, kept alive until the next fence
+        self._inflight = []
 
     # -- construction (GridderState::create, src/hpg.cc:226-264) -----------
     @classmethod
@@ -255,6 +261,7 @@ class GridderState:
         if h:
             _lib.lib().hpgx_destroy(h)
             self._h = None
+        self._inflight = []
 
     def copy(self):
         """Value copy (GridderState copy ctor, src/hpg.cc:266-306)."""
@@ -390,6 +397,8 @@ class GridderState:
         _check(_lib.lib().hpgx_grid_visibilities(
             self._h, ptr, n, npol, int(bool(update_grid_weights)), int(bool(do_degrid)),
             int(bool(return_visibilities)), int(bool(do_grid)), out_ptr, memory), self._h)
+        if memory == 1:
+            self._inflight.append(keep)
         if return_visibilities:
             self.fence()
             return out
@@ -453,7 +462,10 @@ class GridderState:
         return self
 
     def fence(self):
-        _check(_lib.lib().hpgx_fence(self._h), self._h)
+        try:
+            _check(_lib.lib().hpgx_fence(self._h), self._h)
+        finally:
+            self._inflight.clear()
         return self
 
     # -- finishing passes -----------------------------------------------------

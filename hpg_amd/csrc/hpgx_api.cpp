@@ -76,7 +76,7 @@ struct hpgx_state_s {
   // than one workgroup's LDS histogram holds), 3 = block-local bins of one
   // entry per touched tile, 2 = compact entries binned through global
   // counters, 1 = per-visibility records gathered through an index
-  int gridder = 3;
+  int gridder = 4;
   size_t nkeys = 0;
   size_t pass_blocks = 0;  // bin blocks per pass of the block-local pipelines
 
@@ -514,10 +514,13 @@ int upload_model(hpgx_state_t st, const void* values, int layout, int memory) {
 // gridding / degridding pipeline on records already in cb.vis
 // gridding / degridding pipeline on records already in cb.vis; the records
 // are gridded in chunks whose bin entries fit the entry buffer
+// records: device records read in place (nullptr: the staged copy cb.vis)
 int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
-             bool do_grid, hpgx::GvisSource src, const double2* residuals) {
+             bool do_grid, hpgx::GvisSource src, const double2* residuals,
+             const void* records = nullptr) {
   const Geometry geo = st->geometry();
   CallBuffers& cb = st->cb;
+  void* rec_base = records ? const_cast<void*>(records) : cb.vis;
   const int np = int(st->npol);
   bool gvis_ready = false;
   if (do_degrid) {
@@ -538,7 +541,7 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
     for (size_t off = 0; off < n; off += chunk) {
       const size_t m = std::min(chunk, n - off);
       CallBuffers c = cb;
-      c.vis = static_cast<char*>(cb.vis) + off * rec_bytes;
+      c.vis = static_cast<char*>(rec_base) + off * rec_bytes;
       c.gvis = cb.gvis + off * np;
       const double2* res = residuals ? residuals + off * np : nullptr;
       HPGX_HIP(st, hpgx::launch_bin(
@@ -560,7 +563,7 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
   for (size_t off = 0; off < n; off += chunk) {
     const size_t m = std::min(chunk, n - off);
     CallBuffers c = cb;
-    c.vis = static_cast<char*>(cb.vis) + off * rec_bytes;
+    c.vis = static_cast<char*>(rec_base) + off * rec_bytes;
     c.prep = cb.prep + off;
     c.gvis = cb.gvis + off * np;
     const double2* res = residuals ? residuals + off * np : nullptr;
@@ -683,7 +686,7 @@ int hpgx_create(
   st->mode = grid_mode;
   if (const char* g = std::getenv("HPGX_GRIDDER")) {
     const int v = std::atoi(g);
-    st->gridder = (v >= 1 && v <= 4) ? v : 3;
+    st->gridder = (v >= 1 && v <= 4) ? v : 4;
   }
   // Mueller index matrices padded to 4x4 with -1
   // (copy_mueller_indexes, src/hpg_runtime.hpp:1590-1614)
@@ -989,13 +992,17 @@ int hpgx_grid_visibilities(
     return HPGX_SUCCESS;
   }
   if (int rc = ensure_call_buffers(st)) return rc;
-  if (int rc = stage_records(st, vis, n, memory)) return rc;
+  // device records are read in place unless degridding rewrites them
+  const bool in_place = memory == HPGX_DEVICE && !do_degrid;
+  if (!in_place)
+    if (int rc = stage_records(st, vis, n, memory)) return rc;
   if (int rc = run_grid(st, n, update_grid_weights, do_degrid, do_grid,
-                        hpgx::GvisSource::Values, nullptr))
+                        hpgx::GvisSource::Values, nullptr, in_place ? vis : nullptr))
     return rc;
-  if (return_visibilities) {
+  const void* rec_base = in_place ? vis : st->cb.vis;
+  if (return_visibilities && vis_out != rec_base) {
     HPGX_HIP(st, hipMemcpyAsync(
-                   vis_out, st->cb.vis, n * visdata_size(npol),
+                   vis_out, rec_base, n * visdata_size(npol),
                    memory == HPGX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                    st->stream));
   }
@@ -1016,7 +1023,9 @@ int hpgx_degrid_adjoint(
   if (!vis || !residuals) return fail(st, HPGX_ERR_OTHER, "null argument");
   if (int rc = set_device(st)) return rc;
   if (int rc = ensure_call_buffers(st)) return rc;
-  if (int rc = stage_records(st, vis, n, memory)) return rc;
+  const bool in_place = memory == HPGX_DEVICE;
+  if (!in_place)
+    if (int rc = stage_records(st, vis, n, memory)) return rc;
   const double2* res = static_cast<const double2*>(residuals);
   double2* tmp = nullptr;
   if (memory != HPGX_DEVICE) {
@@ -1025,7 +1034,8 @@ int hpgx_degrid_adjoint(
                                 hipMemcpyHostToDevice, st->stream));
     res = tmp;
   }
-  int rc = run_grid(st, n, false, false, true, hpgx::GvisSource::Adjoint, res);
+  int rc = run_grid(st, n, false, false, true, hpgx::GvisSource::Adjoint, res,
+                    in_place ? vis : nullptr);
   if (tmp) {
     (void)hipStreamSynchronize(st->stream);
     dev_free(tmp);

@@ -181,7 +181,10 @@ HPGX_API int hpgx_has_model(hpgx_state_t state);
  * values (do_degrid && do_grid) or predicted values (do_degrid && !do_grid)
  * for records inside the grid -- are written to visibilities_out (same
  * memory kind; may equal visibilities) once the stream reaches that point;
- * they are valid after the next hpgx_fence. */
+ * they are valid after the next hpgx_fence.  HPGX_HOST records are copied
+ * before the call returns; HPGX_DEVICE records are read in place by the
+ * queued kernels unless do_degrid is set, so they must be ready when the
+ * call is made and stay unchanged until the next hpgx_fence. */
 HPGX_API int hpgx_grid_visibilities(
   hpgx_state_t state, const void* visibilities, size_t num_visibilities,
   unsigned num_polarizations, int update_grid_weights, int do_degrid,
@@ -203,7 +206,9 @@ HPGX_API int hpgx_degrid_forward(
  * (num_visibilities x num_polarizations complex<double>, in `memory`) and
  * W_v the degridder's CF normalization, for the records' coordinates.  Adds
  * into the grid; grid weights unchanged.  Exact adjoint when the Mueller
- * index matrices equal their conjugates.  No reference counterpart. */
+ * index matrices equal their conjugates.  HPGX_DEVICE records and residuals
+ * are read in place: unchanged until the next hpgx_fence.  No reference
+ * counterpart. */
 HPGX_API int hpgx_degrid_adjoint(
   hpgx_state_t state, const void* visibilities, const void* residuals,
   size_t num_visibilities, unsigned num_polarizations, int memory);

@@ -135,6 +135,64 @@ def test_random_vs_oracle_fp32_compensated(hpg, oracle_mod, case):
     np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
 
 
+@pytest.mark.parametrize("gridder", [1, 2, 3, 4])
+@pytest.mark.parametrize("case", [0, 2, 3, 4])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_each_gridder_vs_oracle(hpg, oracle_mod, monkeypatch, gridder, case, mode):
+    """Every gridding pipeline (HPGX_GRIDDER, read when a state is created:
+    1 indexed records, 2 globally binned entries, 3 block-local bins with
+    clipped tiles, 4 block-local bins with haloed tiles -- which falls back
+    to 3 for case 3, whose 35-pixel support exceeds a tile) against the
+    oracle, with phase gradients on a third of the records and some records
+    off the grid.  FP64 mode:
+    1e-6 of max|grid| (gradient screens); FP32-compensated: e_max, e_rms
+    <= 1e-6 per plane."""
+    from hpg_amd import workloads
+    monkeypatch.setenv("HPGX_GRIDDER", str(gridder))
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[case]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=40 + case)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (0.9, 1.1), M, cf, mode, Mc)
+    rec = workloads.random_visibilities(1500, npol, gsz, (0.9, 1.1), cf, seed=50 + case,
+                                        phase=True, margin=-3, gradient=True)
+    rec["cf_phase_gradient"][1::3] = 0.0
+    rec["cf_phase_gradient"][2::3] = 0.0
+    print(f"gridder {gridder} case {case} mode {mode}", flush=True)
+    gs.grid_visibilities(hpg.Device.Serial, rec, True)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    got, exp = gs.grid_values(), o.grid_values()
+    for c in range(gsz[3]):
+        for r in range(gsz[2]):
+            if np.abs(exp[c, r]).max() == 0:
+                assert np.abs(got[c, r]).max() == 0
+                continue
+            e_max, e_rms = rel_errors(got[c, r], exp[c, r])
+            assert e_max <= 1e-6 and e_rms <= 1e-6, (c, r, e_max, e_rms)
+    np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
+
+
+@pytest.mark.parametrize("radius", [(15, 2), (2, 15), (16, 1), (0, 0)])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_support_at_tile_edge(hpg, oracle_mod, radius, mode):
+    """Largest supports the haloed tiles take (31 pixels on one axis), the
+    first they hand to the clipped tiles (33), and a 1x1 support (no halo)
+    on a grid whose edges are not tile multiples; records cover the grid up
+    to its edges.  No phase gradients: FP64 grids to 1e-12 of max|grid|,
+    FP32-compensated to 1e-6."""
+    from hpg_amd import workloads
+    gsz = (97, 75, 1, 1)
+    cf = workloads.random_cf(4, [radius], seed=7)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), [[0]], cf, mode)
+    rec = workloads.random_visibilities(2500, 1, gsz, (1.0, 1.0), cf, seed=8, phase=True,
+                                        margin=-2)
+    print(f"support radius {radius} mode {mode}", flush=True)
+    gs.grid_visibilities(hpg.Device.Serial, rec, True)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    got, exp = gs.grid_values(), o.grid_values()
+    e_max, e_rms = rel_errors(got, exp)
+    assert e_max <= (1e-12 if mode == 0 else 1e-6) and e_rms <= 1e-6, (e_max, e_rms)
+    np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
+
+
 def test_serialization_and_batches(hpg, oracle_mod):
     """Serialization (GridderStateUnitTests.cc:1126-1218): the grid does not
     depend on the order of the batches; gridding in batches equals one call."""
