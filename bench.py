@@ -9,12 +9,18 @@ oversampling 20).  A step is one grid_visibilities call over the whole batch,
 with the records already resident in HBM, into the complex<float> grid with
 compensated FP32 accumulation (FP64 weights).  The same step on the
 complex<double> grid is reported beside it, as are the finishing passes
-(normalize_by_weights, FFT, shift) and the relative error of both grids
-against the FP64 CPU grid after normalization and after FFT + shift.
+(normalize_by_weights, FFT, shift), degridding of the same batch from a
+complex<double> model, and the relative error of both grids against the
+FP64 CPU grid after normalization and after FFT + shift.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
-Prints one JSON line.
+Prints one JSON line.  Traffic (HBM bytes per call) comes from two counter
+runs of the same workload, summarized into profiles/:
+
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d F -- python3 bench.py --pmc-workload
+    rocprofv3 --pmc WRITE_SIZE --output-format csv -d W -- python3 bench.py --pmc-workload
+    python3 bench.py --pmc-summary F W > profiles/traffic_grid_1024sq_1e6vis.json
 """
 import argparse
 import json
@@ -84,6 +90,37 @@ def run_gpu(hpg, mode, dev_records, cf, steps, warmup):
     return wall * 1e3 / steps, dev_ms, fin
 
 
+def run_degrid(hpg, dev_records, cf, model, steps, warmup):
+    """Degridding of the same batch from a complex<double> model: calls with
+    do_degrid and without gridding (predicted values written into the
+    state's copy of the records), timed on the state's stream.  Returns
+    device ms per call."""
+    gs = hpg.GridderState.create(hpg.Device.Hip, 0, NUM_VIS, cf, (GRID, GRID, 1, 1),
+                                 (1.0, 1.0), [[0]], grid_mode=hpg.GridMode.FP32_COMPENSATED)
+    gs.set_convolution_function(hpg.Device.Serial, cf)
+    gs.set_model(hpg.Device.Serial, model)
+    for _ in range(warmup):
+        gs.grid_visibilities_base(hpg.Device.Serial, dev_records, False, True, False, False)
+    gs.fence()
+    gs.event_record(6)
+    for _ in range(steps):
+        gs.grid_visibilities_base(hpg.Device.Serial, dev_records, False, True, False, False)
+    gs.event_record(7)
+    gs.fence()
+    ms = gs.event_elapsed_ms(6, 7) / steps
+    gs.release()
+    return ms
+
+
+def degrid_bytes(num_vis):
+    """Compulsory HBM bytes of one degridding call: the records read, the
+    predicted values written back (8 bytes each), the CF and the FP64
+    model read once."""
+    support = 2 * CF_RADIUS + 1
+    cf = (support + 4) ** 2 * CF_OS ** 2 * 8
+    return num_vis * (88 + 8) + cf + GRID * GRID * 16
+
+
 def precision_grids(hpg, mode, dev_records, cf):
     """One call on a fresh grid; returns the normalized grid and the grid after
     FFT + shift (complex128, 1024x1024)."""
@@ -98,6 +135,117 @@ def precision_grids(hpg, mode, dev_records, cf):
     g_img = gs.grid_values()[0, 0]
     gs.release()
     return g_norm, g_img
+
+
+# HBM traffic per call from rocprofv3 counters (FETCH_SIZE and WRITE_SIZE,
+# one run each, calibrated by a plain copy of known size); written by
+# --pmc-summary, read by the default run
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_grid_1024sq_1e6vis.json")
+CAL_BYTES = 256 << 20
+
+
+def pmc_workload(hpg, dev_records, cf):
+    """The program profiled by the two counter runs: a 256 MiB device copy
+    (calibration), then 3 gridding calls into each grid type."""
+    import torch
+    a = torch.ones(CAL_BYTES, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    for mode in (hpg.GridMode.FP32_COMPENSATED, hpg.GridMode.FP64):
+        gs = hpg.GridderState.create(hpg.Device.Hip, 0, NUM_VIS, cf, (GRID, GRID, 1, 1),
+                                     (1.0, 1.0), [[0]], grid_mode=mode)
+        gs.set_convolution_function(hpg.Device.Serial, cf)
+        for _ in range(3):
+            gs.grid_visibilities(hpg.Device.Serial, dev_records)
+        gs.fence()
+        gs.release()
+    print(json.dumps({"pmc_workload": "done", "calibration_bytes": CAL_BYTES}), flush=True)
+
+
+def read_counters(directory):
+    """{kernel name: [per-dispatch value]} from a rocprofv3 counter CSV."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {directory}")
+    out = {}
+    for f in files:
+        for row in csv.DictReader(open(f)):
+            name = row.get("Kernel_Name", "")
+            out.setdefault(name, []).append(float(row["Counter_Value"]))
+    return out
+
+
+def short_name(kernel):
+    import re
+    k = kernel.replace("void ", "").replace("hpgx::", "").replace("(anonymous namespace)::", "")
+    return re.sub(r"\(.*", "", k).strip()
+
+
+def pmc_summary(fetch_dir, write_dir):
+    """Per-call HBM traffic of each grid type's gridding kernels, in bytes:
+    counter KiB x 1024 x (calibration bytes / what the counter reads for
+    the calibration copy)."""
+    import numpy as np
+    res = {"calibration_bytes": CAL_BYTES, "kernels": {}}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        vals = read_counters(d)
+        cal = max(v for k, vs in vals.items() if "copyBuffer" in k for v in vs)
+        factor = CAL_BYTES / (cal * 1024.0)
+        res[counter + "_calibration_kib"] = cal
+        res[counter + "_factor"] = factor
+        for k, vs in vals.items():
+            if "copyBuffer" in k or not short_name(k).startswith("k_"):
+                continue
+            e = res["kernels"].setdefault(short_name(k), {})
+            e[counter + "_bytes"] = float(np.median(vs)) * 1024.0 * factor
+            e["dispatches"] = len(vs)
+    per_call = {"fp32c": 0.0, "fp64": 0.0}
+    for k, e in res["kernels"].items():
+        b = e.get("FETCH_SIZE_bytes", 0.0) + e.get("WRITE_SIZE_bytes", 0.0)
+        if k.startswith("k_bin"):
+            per_call["fp32c"] += b
+            per_call["fp64"] += b
+        elif k.startswith("k_tile4<1, 1") or k.startswith("k_halo<HIP_vector_type<float"):
+            per_call["fp32c"] += b
+        elif k.startswith("k_tile4<1, 0") or k.startswith("k_halo<HIP_vector_type<double"):
+            per_call["fp64"] += b
+    res["per_call_bytes"] = per_call
+    print(json.dumps(res, indent=1), flush=True)
+
+
+def traffic_per_call(grid):
+    """Measured HBM bytes per call for 'fp32c' or 'fp64', or None."""
+    try:
+        t = json.load(open(TRAFFIC_FILE))
+        return t["per_call_bytes"][grid]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def copy_bandwidth():
+    """Achievable HBM bandwidth: a 2 GiB device-to-device copy, GB/s
+    (read + write bytes / time)."""
+    import torch
+    n = 2 << 30
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n / (ms * 1e-3) / 1e9
 
 
 def rel_err(got, ref):
@@ -154,7 +302,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline and precision legs")
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="run the program profiled by the counter runs and exit")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarize the two counter runs' CSVs and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -170,10 +325,20 @@ def main():
     records = workloads.config1_visibilities(NUM_VIS, GRID)
     dev_records = torch.from_numpy(records.view(np.uint8).copy()).cuda()
     torch.cuda.synchronize()
+    if args.pmc_workload:
+        pmc_workload(hpg, dev_records, cf)
+        return
+    copy_gbs = copy_bandwidth()
 
     results = {}
     for name, mode in [("fp32c", hpg.GridMode.FP32_COMPENSATED), ("fp64", hpg.GridMode.FP64)]:
         results[name] = run_gpu(hpg, mode, dev_records, cf, args.steps, args.warmup)
+
+    rng = np.random.default_rng(7)
+    model = (rng.standard_normal((1, 1, GRID, GRID))
+             + 1j * rng.standard_normal((1, 1, GRID, GRID)))
+    degrid_ms = run_degrid(hpg, dev_records, cf, model, max(1, args.steps // 4),
+                           max(1, args.warmup // 4))
 
     wall_ms, dev_ms, fin = results["fp32c"]
     nbytes = algorithmic_bytes(NUM_VIS, 8)
@@ -198,7 +363,9 @@ def main():
                    "cf_support": 2 * CF_RADIUS + 1, "oversampling": CF_OS,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_per_call("fp32c"),
+                     "copy_kernel_gbs": copy_gbs,
                      "kernel": "grid_visibilities call (k_bin + k_tile4 + k_halo)",
                      "bytes_per_call": nbytes, "device_ms_per_call": dev_ms,
                      "fp32_frac": flops / (dev_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS},
@@ -208,7 +375,13 @@ def main():
                       "device_ms_per_call": results["fp64"][1],
                       "hbm_frac": algorithmic_bytes(NUM_VIS, 16)
                       / (results["fp64"][1] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "traffic": traffic_per_call("fp64"),
                       "finishing_ms": results["fp64"][2]},
+        "degrid": {"vis_per_s": NUM_VIS / (degrid_ms * 1e-3),
+                   "device_ms_per_call": degrid_ms,
+                   "bytes_per_call": degrid_bytes(NUM_VIS),
+                   "hbm_frac": degrid_bytes(NUM_VIS) / (degrid_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                   "kernel": "record staging copy + k_degrid_lane"},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -79,6 +79,10 @@ struct hpgx_state_s {
   int gridder = 4;
   size_t nkeys = 0;
   size_t pass_blocks = 0;  // bin blocks per pass of the block-local pipelines
+  // tile accumulation of FP32 grids in the haloed-tile gridder: 2 = FP32
+  // hi/lo pairs updated by two-sum under compare-and-swap (default), 1 = FP32
+  // two-sum through returning FP32 LDS atomics, 0 = FP64 tile sums
+  int fp32_tile = 2;
 
   Plan plan_z2z, plan_c2c;
   hipEvent_t events[num_event_slots] = {};
@@ -549,7 +553,7 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
                      res, c, block_cap, st->weights, halo));
       if (halo)
         HPGX_HIP(st, hpgx::launch_tile_halo(
-                       st->stream, np, mode, m, st->nkeys, geo, st->cf, st->mu, c,
+                       st->stream, np, mode, st->fp32_tile, m, st->nkeys, geo, st->cf, st->mu, c,
                        block_cap, halo_x(st), halo_y(st), st->grid));
       else
         HPGX_HIP(st, hpgx::launch_tile_blocal(
@@ -684,6 +688,10 @@ int hpgx_create(
   st->gscale[1] = grid_scale[1];
   st->npol = num_polarizations;
   st->mode = grid_mode;
+  if (const char* a = std::getenv("HPGX_FP32_TILE")) {
+    const std::string v(a);
+    st->fp32_tile = v == "fp64" ? 0 : v == "rtn" ? 1 : 2;
+  }
   if (const char* g = std::getenv("HPGX_GRIDDER")) {
     const int v = std::atoi(g);
     st->gridder = (v >= 1 && v <= 4) ? v : 4;
@@ -739,6 +747,7 @@ int hpgx_clone(hpgx_state_t src, hpgx_state_t* dst) {
     src->mode, 0, 0, nullptr);
   if (rc) return fail(src, rc, "clone: " + std::string(g_create_error));
   st->gridder = src->gridder;
+  st->fp32_tile = src->fp32_tile;
   auto copy = [&](void* d, const void* s, size_t bytes) {
     return hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, st->stream);
   };
@@ -1067,9 +1076,14 @@ int hpgx_degrid_forward(
   HPGX_HIP(st, hipMemsetAsync(out, 0, bytes, st->stream));
   if (st->cf.ngroups > 0) {
     if (int rc = ensure_call_buffers(st)) return rc;
-    if (int rc = stage_records(st, vis, n, memory)) return rc;
+    // device records are read in place by the lane-per-record degridder
+    const bool in_place = memory == HPGX_DEVICE
+      && size_t(st->cf.max_sx) * size_t(st->cf.max_sy) <= hpgx::degrid_lane_max_taps();
+    if (!in_place)
+      if (int rc = stage_records(st, vis, n, memory)) return rc;
     HPGX_HIP(st, hpgx::launch_degrid(st->stream, int(npol), n, st->geometry(), st->cf,
-                                     st->mu, st->model, false, st->cb, out));
+                                     st->mu, st->model, false, st->cb, out,
+                                     in_place ? vis : nullptr));
   }
   if (tmp) {
     HPGX_HIP(st, hipMemcpyAsync(predictions, tmp, bytes, hipMemcpyDeviceToHost, st->stream));

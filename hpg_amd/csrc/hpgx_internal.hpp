@@ -151,18 +151,24 @@ size_t halo_entry_bytes(int npol);
 size_t halo_pixels_per_key(int hx, int hy);
 int halo_tile_limit();
 size_t halo_lds_bytes(int hx, int hy, int nbins);
+// acc (FP32 grids): 0 FP64 tile sums, 1 FP32 two-sum via returning atomics,
+// 2 FP32 two-sum via compare-and-swap of packed (hi, lo) pairs
 hipError_t launch_tile_halo(
-  hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
+  hipStream_t s, int npol, int mode, int acc, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
   int hx, int hy, void* grid);
 hipError_t launch_tile_blocal(
   hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
   void* grid);
+// degridding; with pred64 the FP64 predictions are written there and the
+// records are only read, from `records` when given (else cb.vis; `records`
+// requires supports of at most degrid_lane_max_taps() taps)
+size_t degrid_lane_max_taps();
 hipError_t launch_degrid(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, const double2* model, bool do_grid, CallBuffers& cb,
-  double2* pred64 = nullptr);
+  double2* pred64 = nullptr, const void* records = nullptr);
 
 hipError_t launch_normalize(
   hipStream_t s, int mode, void* grid, const double* weights, double wfactor,

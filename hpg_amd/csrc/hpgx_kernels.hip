@@ -941,6 +941,19 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
 
   const int ntiles = geo.ntx * geo.nty;
   const size_t b = blockIdx.x;
+  // weights of this thread's records, per grid row, for one grid cube at a
+  // time (records of another cube flush the sums first)
+  int wcube = -1;
+  double wacc[4] = {0.0, 0.0, 0.0, 0.0};
+  auto flush_weights = [&](int cube, const double* acc) {
+    for (int gpol = 0; gpol < geo.nr; ++gpol) {
+      const int k = cube * geo.nr + gpol;
+      if (local_w)
+        __hip_atomic_fetch_add(wl + k, acc[gpol], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      else
+        atomicAdd(weights + k, acc[gpol]);
+    }
+  };
   Entry<N> e[bin_recs_per_thread];
   bool keep[bin_recs_per_thread];
   unsigned short sxy[bin_recs_per_thread][2];
@@ -1010,12 +1023,15 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
           }
           t += mag * double(r.w[v]);
         }
-        double* dst = local_w ? wl + gcube * geo.nr + gpol
-                              : weights + size_t(gcube) * geo.nr + gpol;
-        if (local_w)
-          __hip_atomic_fetch_add(dst, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        else
-          atomicAdd(dst, t);
+        if (int(gcube) != wcube) {
+          if (wcube >= 0) flush_weights(wcube, wacc);
+          wcube = int(gcube);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) wacc[k] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (k == gpol) wacc[k] += t;
       }
     }
     const int tx0 = tiles_begin(c.gx), tx1 = HALO ? tx0 + 1 : tiles_end(c.gx, g.sx);
@@ -1027,6 +1043,23 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
         for (int tx = tx0; tx < tx1; ++tx)
           __hip_atomic_fetch_add(cnt + base + ty * geo.ntx + tx, 1u, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  // the threads' weight sums: one sum per (wave, grid cube) present, the
+  // wave's lanes all active here
+  if (update_weights) {
+    const int lane = threadIdx.x % wave;
+    int key = wcube;
+    for (;;) {
+      const unsigned long long act = __ballot(key >= 0);
+      if (act == 0) break;
+      const int leader = __ffsll((long long)act) - 1;
+      const int lk = __shfl(key, leader, wave);
+      double part_w[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) part_w[k] = wave_sum(key == lk ? wacc[k] : 0.0);
+      if (lane == leader) flush_weights(lk, part_w);
+      if (key == lk) key = -1;
     }
   }
   // invalid records: one global atomic per wave at most
@@ -1326,7 +1359,42 @@ __host__ __device__ inline int halo_pixels(int hx, int hy) {
   return hx * tile + (tile + hx) * hy;
 }
 
-template <int N, int MODE>
+// tile accumulation schemes of k_tile4
+//   acc_f64       FP64 LDS atomics (FP64 grids; also selectable for FP32 grids)
+//   acc_f32_rtn   FP32 hi/lo per pixel: hi via FP32 LDS atomics returning the
+//                 old value, its exact rounding error (two-sum) into lo
+//   acc_f32_cas   FP32 hi/lo per pixel packed into 64 bits and updated by
+//                 compare-and-swap: hi' = fl(hi + p), lo' = fl(lo + err) with
+//                 err the exact two-sum error, so each update is one
+//                 error-free transformation on one coherent pair
+constexpr int acc_f64 = 0;
+constexpr int acc_f32_rtn = 1;
+constexpr int acc_f32_cas = 2;
+
+__device__ __forceinline__ unsigned long long pack_pair(float hi, float lo) {
+  return (unsigned long long)__float_as_uint(hi)
+    | ((unsigned long long)__float_as_uint(lo) << 32);
+}
+
+// adds h (+ the low-order part l) into the packed pair at a, starting from
+// the expected value `expect`; loops only while other lanes or waves change
+// the pair between the read and the swap, each retry following one of their
+// successful updates
+__device__ __forceinline__ void pair_add(
+  unsigned long long* a, unsigned long long expect, float h, float l) {
+  for (;;) {
+    const float hi = __uint_as_float(unsigned(expect));
+    const float lo = __uint_as_float(unsigned(expect >> 32));
+    float sum, err;
+    two_sum(hi, h, sum, err);
+    const unsigned long long nw = pack_pair(sum, lo + (err + l));
+    if (__hip_atomic_compare_exchange_strong(a, &expect, nw, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+      return;
+  }
+}
+
+template <int N, int MODE, int ACC>
 __global__ __launch_bounds__(halo_block) void k_tile4(
   Geometry geo, const GroupInfo* __restrict__ groups,
   const float2* __restrict__ taps, Mueller mu, const unsigned* __restrict__ off_table,
@@ -1334,14 +1402,16 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
   const unsigned* __restrict__ entry_vis, const VisRec<N>* __restrict__ recs,
   int hx, int hy, void* __restrict__ grid, double2* __restrict__ halo,
   unsigned* __restrict__ halo_set) {
+  static_assert(MODE == 1 || ACC == acc_f64, "FP64 grids accumulate in FP64");
 
   extern __shared__ double smem[];
   const int RX = tile + hx;
   const int RY = tile + hy;
   const int R = RX * RY;
   // LDS: accumulators (16 bytes per region pixel), then the segment tables
-  double* acc64 = smem;                                // MODE 0: re[R], im[R]
-  float* acc32 = reinterpret_cast<float*>(smem);       // MODE 1: hr, hi, lr, li
+  double* acc64 = smem;                                // acc_f64: re[R], im[R]
+  float* acc32 = reinterpret_cast<float*>(smem);       // acc_f32_rtn: hr, hi, lr, li
+  unsigned long long* pairs = reinterpret_cast<unsigned long long*>(smem);  // cas: re[R], im[R]
   unsigned* seg_pref = reinterpret_cast<unsigned*>(smem + 2 * R);
   unsigned* seg_beg = seg_pref + nbins + 1;
   unsigned* part = seg_beg + nbins;
@@ -1430,36 +1500,26 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
           for (int v = 0; v < N; ++v)
             c[v][k] = (X0 + k < sx && use[v]) ? slice[v][Y * sx + X0 + k]
                                               : make_float2(0.0f, 0.0f);
+        // products of the chunk's taps: hi parts (FP32) and, for N > 1, the
+        // low-order parts of their FP64 sums over the polarizations
+        float hr[halo_chunk], him[halo_chunk], lr[halo_chunk], lim[halo_chunk];
+        double dr[halo_chunk], di[halo_chunk];
 #pragma unroll
         for (int k = 0; k < halo_chunk; ++k) {
-          if (X0 + k >= sx) break;
           const int X = X0 + k;
           float2 scr = make_float2(1.0f, 0.0f);
-          if (grad) scr = narrow(cphase((phi0x + X * dphix) + (phi0y + Y * dphiy)));
-          const int li = lrow + X;
+          if (grad && X < sx) scr = narrow(cphase((phi0x + X * dphix) + (phi0y + Y * dphiy)));
           if constexpr (N == 1) {
             float2 cc = c[0][k];
             cc.y *= imf;
             if (grad) cc = cmul(cc, scr);
             const float2 pr = use[0] ? cmul(cc, e.g[0]) : make_float2(0.0f, 0.0f);
-            if constexpr (MODE == 0) {
-              __hip_atomic_fetch_add(acc64 + li, double(pr.x), __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-              __hip_atomic_fetch_add(acc64 + R + li, double(pr.y), __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-            } else {
-              const float orr = __hip_atomic_fetch_add(
-                acc32 + li, pr.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              const float ori = __hip_atomic_fetch_add(
-                acc32 + R + li, pr.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              float s, er, ei;
-              two_sum(orr, pr.x, s, er);
-              two_sum(ori, pr.y, s, ei);
-              __hip_atomic_fetch_add(acc32 + 2 * R + li, er, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-              __hip_atomic_fetch_add(acc32 + 3 * R + li, ei, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
+            hr[k] = pr.x;
+            him[k] = pr.y;
+            lr[k] = 0.0f;
+            lim[k] = 0.0f;
+            dr[k] = double(pr.x);
+            di[k] = double(pr.y);
           } else {
             double sr = 0.0, si = 0.0;
 #pragma unroll
@@ -1472,26 +1532,58 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
               sr += double(pr.x);
               si += double(pr.y);
             }
-            if constexpr (MODE == 0) {
-              __hip_atomic_fetch_add(acc64 + li, sr, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-              __hip_atomic_fetch_add(acc64 + R + li, si, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-            } else {
-              const float hr = float(sr), him = float(si);
-              const float lr = float(sr - double(hr)), lim = float(si - double(him));
-              const float orr = __hip_atomic_fetch_add(
-                acc32 + li, hr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              const float ori = __hip_atomic_fetch_add(
-                acc32 + R + li, him, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              float s, er, ei;
-              two_sum(orr, hr, s, er);
-              two_sum(ori, him, s, ei);
-              __hip_atomic_fetch_add(acc32 + 2 * R + li, er + lr, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-              __hip_atomic_fetch_add(acc32 + 3 * R + li, ei + lim, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
+            dr[k] = sr;
+            di[k] = si;
+            hr[k] = float(sr);
+            him[k] = float(si);
+            lr[k] = float(sr - double(hr[k]));
+            lim[k] = float(si - double(him[k]));
+          }
+        }
+        if constexpr (ACC == acc_f64) {
+#pragma unroll
+          for (int k = 0; k < halo_chunk; ++k) {
+            if (X0 + k >= sx) break;
+            const int li = lrow + X0 + k;
+            __hip_atomic_fetch_add(acc64 + li, dr[k], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(acc64 + R + li, di[k], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        } else if constexpr (ACC == acc_f32_rtn) {
+#pragma unroll
+          for (int k = 0; k < halo_chunk; ++k) {
+            if (X0 + k >= sx) break;
+            const int li = lrow + X0 + k;
+            const float orr = __hip_atomic_fetch_add(
+              acc32 + li, hr[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const float ori = __hip_atomic_fetch_add(
+              acc32 + R + li, him[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            float sum, er, ei;
+            two_sum(orr, hr[k], sum, er);
+            two_sum(ori, him[k], sum, ei);
+            __hip_atomic_fetch_add(acc32 + 2 * R + li, er + lr[k], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(acc32 + 3 * R + li, ei + lim[k], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        } else {
+          // read the chunk's pairs first, then swap each from that value
+          unsigned long long er[halo_chunk], ei[halo_chunk];
+#pragma unroll
+          for (int k = 0; k < halo_chunk; ++k) {
+            if (X0 + k >= sx) break;
+            const int li = lrow + X0 + k;
+            er[k] = __hip_atomic_load(pairs + li, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            ei[k] = __hip_atomic_load(pairs + R + li, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+#pragma unroll
+          for (int k = 0; k < halo_chunk; ++k) {
+            if (X0 + k >= sx) break;
+            const int li = lrow + X0 + k;
+            pair_add(pairs + li, er[k], hr[k], lr[k]);
+            pair_add(pairs + R + li, ei[k], him[k], lim[k]);
           }
         }
       }
@@ -1504,18 +1596,27 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
   for (int k = threadIdx.x; k < R; k += halo_block) {
     const int ry = k / RX;
     const int rx = k - ry * RX;
+    // tile value: FP64 sum (acc_f64) or FP32 hi + lo
     double vr, vi;
-    float fr = 0.0f, fi = 0.0f, lr = 0.0f, li = 0.0f;
-    if constexpr (MODE == 0) {
+    float fr = 0.0f, fi = 0.0f, flr = 0.0f, fli = 0.0f;
+    if constexpr (ACC == acc_f64) {
       vr = acc64[k];
       vi = acc64[R + k];
     } else {
-      fr = acc32[k];
-      fi = acc32[R + k];
-      lr = acc32[2 * R + k];
-      li = acc32[3 * R + k];
-      vr = double(fr) + double(lr);
-      vi = double(fi) + double(li);
+      if constexpr (ACC == acc_f32_rtn) {
+        fr = acc32[k];
+        fi = acc32[R + k];
+        flr = acc32[2 * R + k];
+        fli = acc32[3 * R + k];
+      } else {
+        const unsigned long long pr = pairs[k], pi = pairs[R + k];
+        fr = __uint_as_float(unsigned(pr));
+        flr = __uint_as_float(unsigned(pr >> 32));
+        fi = __uint_as_float(unsigned(pi));
+        fli = __uint_as_float(unsigned(pi >> 32));
+      }
+      vr = double(fr) + double(flr);
+      vi = double(fi) + double(fli);
     }
     if (rx < tile && ry < tile) {
       const int xx = x0 + rx, yy = y0 + ry;
@@ -1527,13 +1628,18 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
         gvl.x += vr;
         gvl.y += vi;
         *gp = gvl;
+      } else if constexpr (ACC == acc_f64) {
+        // one rounding of grid value + FP64 tile sum
+        float2* gp = static_cast<float2*>(grid) + gi;
+        const float2 g0 = *gp;
+        *gp = make_float2(float(double(g0.x) + vr), float(double(g0.y) + vi));
       } else {
         float2* gp = static_cast<float2*>(grid) + gi;
         const float2 g0 = *gp;
         float sr, er, si, ei;
         two_sum(g0.x, fr, sr, er);
         two_sum(g0.y, fi, si, ei);
-        *gp = make_float2(sr + (er + lr), si + (ei + li));
+        *gp = make_float2(sr + (er + flr), si + (ei + fli));
       }
     } else {
       const int h = ry < tile ? ry * hx + (rx - tile) : tile * hx + (ry - tile) * RX + rx;
@@ -1705,6 +1811,133 @@ __global__ __launch_bounds__(block) void k_degrid(
       } else {
         r.vis[v] = pred[v];
       }
+    }
+  }
+}
+
+// One lane per visibility (supports up to degrid_lane_taps taps): the lane
+// walks the footprint in chunks of halo_chunk taps, issuing the chunk's
+// model-pixel and CF-tap loads together, and accumulates in FP64 registers;
+// no cross-lane reduction.  Same per-tap arithmetic as k_degrid.
+constexpr int degrid_lane_taps = 1024;
+
+template <int N>
+__global__ __launch_bounds__(block) void k_degrid_lane(
+  size_t n, Geometry geo, const GroupInfo* __restrict__ groups,
+  const float2* __restrict__ taps, const double2* __restrict__ wsum, Mueller mu,
+  const double2* __restrict__ model, int do_grid, const VisRec<N>* recs_in,
+  VisRec<N>* recs, float2* __restrict__ gvis, double2* __restrict__ pred64) {
+  // recs_in and recs may be the same records: each lane reads its record's
+  // fields before writing them back
+
+  const size_t i = blockIdx.x * (size_t)block + threadIdx.x;
+  if (i >= n) return;
+  const VisRec<N>& r = recs_in[i];
+  const unsigned grp = r.cf_index[1];
+  const unsigned cfcube = r.cf_index[0];
+  const unsigned gcube = r.grid_cube;
+  if (!(grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc))) return;
+  const GroupInfo g = groups[grp];
+  if (!(cfcube < unsigned(g.ncube))) return;
+  const Coord c = vis_coord(r, geo, g);
+  if (!c.inside) return;  // values left unchanged (src/hpg_core.hpp:1136-1169)
+  const bool pos_w = r.uvw[2] > 0;
+  const double2 phasor = cphase(r.phase);
+  const int fxi = c.fx + geo.F;
+  const int fyi = c.fy + geo.F;
+
+  float2 pred[N];
+  double2 pred_d[N];
+  for (int v = 0; v < N; ++v) {
+    pred[v] = make_float2(0.0f, 0.0f);
+    pred_d[v] = make_double2(0.0, 0.0);
+  }
+  if (model) {
+    const float imf = pos_w ? 1.0f : -1.0f;
+    const bool grad = r.grad[0] != 0.0 || r.grad[1] != 0.0;
+    const double phi0x = -r.grad[0] * ((g.sx / 2) * geo.os - c.fx);
+    const double phi0y = -r.grad[1] * ((g.sy / 2) * geo.os - c.fy);
+    const double dphix = r.grad[0] * geo.os;
+    const double dphiy = r.grad[1] * geo.os;
+    const int sx = g.sx, sy = g.sy;
+    const long long slice_taps = (long long)sx * sy;
+    double2 a[N];
+    for (int v = 0; v < N; ++v) a[v] = make_double2(0.0, 0.0);
+    for (int gpol = 0; gpol < geo.nr; ++gpol) {
+      const int* row = pos_w ? mu.Mc[gpol] : mu.M[gpol];
+      const float2* slice[N];
+      bool use[N];
+      bool any = false;
+      for (int v = 0; v < N; ++v) {
+        const int m = row[v];
+        use[v] = m >= 0 && m < g.nmueller;
+        any |= use[v];
+        slice[v] = taps + g.tap_offset
+          + (((long long)(cfcube * g.nmueller + (use[v] ? m : 0)) * geo.nf + fxi) * geo.nf
+             + fyi) * slice_taps;
+      }
+      if (!any) continue;
+      const double2* mplane =
+        model + (size_t(gcube) * geo.nr + gpol) * size_t(geo.ny) * geo.nx;
+      for (int Y = 0; Y < sy; ++Y) {
+        const double2* mrow = mplane + size_t(c.gy + Y) * geo.nx + c.gx;
+        for (int X0 = 0; X0 < sx; X0 += halo_chunk) {
+          double2 mv[halo_chunk];
+          float2 cf[N][halo_chunk];
+#pragma unroll
+          for (int k = 0; k < halo_chunk; ++k) {
+            const bool in = X0 + k < sx;
+            mv[k] = in ? mrow[X0 + k] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int v = 0; v < N; ++v)
+              cf[v][k] = (in && use[v]) ? slice[v][Y * sx + X0 + k] : make_float2(0.0f, 0.0f);
+          }
+#pragma unroll
+          for (int k = 0; k < halo_chunk; ++k) {
+            if (X0 + k >= sx) break;
+            const int X = X0 + k;
+            double2 m = mv[k];
+            if (grad) {
+              const double phX = phi0x + X * dphix;
+              const double phY = phi0y + Y * dphiy;
+              m = cmul(m, cphase(-phX - phY));
+            }
+#pragma unroll
+            for (int v = 0; v < N; ++v) {
+              if (!use[v]) continue;
+              float2 cc = cf[v][k];
+              cc.y *= imf;
+              const double2 p = cmul(cc, m);
+              a[v].x += p.x;
+              a[v].y += p.y;
+            }
+          }
+        }
+      }
+    }
+    double2 W[N];
+    degrid_norms<N>(geo, g, wsum, mu, pos_w, cfcube, fxi, fyi, W);
+    const double2 conj_phasor = make_double2(phasor.x, -phasor.y);
+    for (int v = 0; v < N; ++v) {
+      const double2 d = (W[v].x != 0.0 || W[v].y != 0.0) ? W[v] : make_double2(1.0, 0.0);
+      pred_d[v] = cmul(kdiv(a[v], d), conj_phasor);
+      pred[v] = narrow(pred_d[v]);
+    }
+  }
+  if (pred64) {
+    for (int v = 0; v < N; ++v) pred64[i * N + v] = pred_d[v];
+    return;
+  }
+  VisRec<N>& o = recs[i];
+  for (int v = 0; v < N; ++v) {
+    if (do_grid) {
+      float2 val = r.vis[v];
+      val.x -= pred[v].x;
+      val.y -= pred[v].y;
+      o.vis[v] = val;
+      gvis[i * N + v] = prephase(val, phasor, r.w[v]);
+    } else {
+      o.vis[v] = pred[v];
     }
   }
 }
@@ -2031,7 +2264,7 @@ size_t halo_lds_bytes(int hx, int hy, int nbins) {
 }
 
 hipError_t launch_tile_halo(
-  hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
+  hipStream_t s, int npol, int mode, int acc, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
   int hx, int hy, void* grid) {
   if (nkeys == 0 || n == 0) return hipSuccess;
@@ -2039,29 +2272,30 @@ hipError_t launch_tile_halo(
   const int nb = bin_blocks_for(n);
   const size_t lds = halo_lds_bytes(hx, hy, nb);
   if (lds > 65536) return hipErrorInvalidValue;
-#define HPGX_TILE4(NP, MD)                                                     \
+#define HPGX_TILE4(NP, MD, AC)                                                 \
   hipLaunchKernelGGL(                                                          \
-    (k_tile4<NP, MD>), dim3(unsigned(nkeys)), dim3(halo_block), lds, s, geo,   \
-    cf.groups, cf.taps, mu, cb.off_table, nb, block_cap,                       \
+    (k_tile4<NP, MD, AC>), dim3(unsigned(nkeys)), dim3(halo_block), lds, s,    \
+    geo, cf.groups, cf.taps, mu, cb.off_table, nb, block_cap,                  \
     static_cast<const Entry4<NP>*>(cb.stream_entries), cb.entry_vis,           \
     static_cast<const VisRec<NP>*>(cb.vis), hx, hy, grid, cb.halo, cb.halo_set)
-  if (mode == 0) {
-    switch (npol) {
-    case 1: HPGX_TILE4(1, 0); break;
-    case 2: HPGX_TILE4(2, 0); break;
-    case 3: HPGX_TILE4(3, 0); break;
-    case 4: HPGX_TILE4(4, 0); break;
-    default: return hipErrorInvalidValue;
-    }
-  } else {
-    switch (npol) {
-    case 1: HPGX_TILE4(1, 1); break;
-    case 2: HPGX_TILE4(2, 1); break;
-    case 3: HPGX_TILE4(3, 1); break;
-    case 4: HPGX_TILE4(4, 1); break;
-    default: return hipErrorInvalidValue;
-    }
+#define HPGX_TILE4_NP(MD, AC)                                                  \
+  switch (npol) {                                                              \
+  case 1: HPGX_TILE4(1, MD, AC); break;                                        \
+  case 2: HPGX_TILE4(2, MD, AC); break;                                        \
+  case 3: HPGX_TILE4(3, MD, AC); break;                                        \
+  case 4: HPGX_TILE4(4, MD, AC); break;                                        \
+  default: return hipErrorInvalidValue;                                        \
   }
+  if (mode == 0) {
+    HPGX_TILE4_NP(0, acc_f64)
+  } else if (acc == 0) {
+    HPGX_TILE4_NP(1, acc_f64)
+  } else if (acc == 1) {
+    HPGX_TILE4_NP(1, acc_f32_rtn)
+  } else {
+    HPGX_TILE4_NP(1, acc_f32_cas)
+  }
+#undef HPGX_TILE4_NP
 #undef HPGX_TILE4
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -2112,11 +2346,33 @@ hipError_t launch_tile_blocal(
   return hipGetLastError();
 }
 
+size_t degrid_lane_max_taps() { return size_t(degrid_lane_taps); }
+
 hipError_t launch_degrid(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, const double2* model, bool do_grid, CallBuffers& cb,
-  double2* pred64) {
+  double2* pred64, const void* records) {
   if (n == 0) return hipSuccess;
+  const void* in = records ? records : cb.vis;
+  if (size_t(cf.max_sx) * size_t(cf.max_sy) <= size_t(degrid_lane_taps)) {
+#define HPGX_DEGRID_LANE(NP)                                                   \
+  hipLaunchKernelGGL(                                                          \
+    k_degrid_lane<NP>, dim3(blocks_for(n)), dim3(block), 0, s, n, geo,         \
+    cf.groups, cf.taps, cf.wsum, mu, model, int(do_grid),                      \
+    static_cast<const VisRec<NP>*>(in), static_cast<VisRec<NP>*>(cb.vis),      \
+    cb.gvis, pred64)
+    switch (npol) {
+    case 1: HPGX_DEGRID_LANE(1); break;
+    case 2: HPGX_DEGRID_LANE(2); break;
+    case 3: HPGX_DEGRID_LANE(3); break;
+    case 4: HPGX_DEGRID_LANE(4); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef HPGX_DEGRID_LANE
+    return hipGetLastError();
+  }
+  if (records) return hipErrorInvalidValue;  // the wave form reads cb.vis
+  // (callers stage records for supports above degrid_lane_max_taps())
   const unsigned nb = unsigned((n + (block / wave) - 1) / (block / wave));
 #define HPGX_DEGRID(NP)                                                        \
   hipLaunchKernelGGL(                                                          \

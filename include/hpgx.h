@@ -195,7 +195,8 @@ HPGX_API int hpgx_grid_visibilities(
  * their rounding to complex<float>: predictions are num_visibilities x
  * num_polarizations complex<double>, in `memory`.  The records are not
  * modified and the grid is untouched.  Same arithmetic as the degridding of
- * hpgx_grid_visibilities (degrid_vis, src/hpg_core.hpp:579-682).  No
+ * hpgx_grid_visibilities (degrid_vis, src/hpg_core.hpp:579-682).  HPGX_DEVICE
+ * records are read in place: unchanged until the next hpgx_fence.  No
  * reference counterpart (the reference has no differentiable form). */
 HPGX_API int hpgx_degrid_forward(
   hpgx_state_t state, const void* visibilities, void* predictions,

@@ -170,6 +170,33 @@ def test_each_gridder_vs_oracle(hpg, oracle_mod, monkeypatch, gridder, case, mod
     np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
 
 
+@pytest.mark.parametrize("scheme", ["cas", "rtn", "fp64"])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_fp32_tile_schemes(hpg, oracle_mod, monkeypatch, scheme, case):
+    """Each tile accumulation of the FP32 grid (HPGX_FP32_TILE: FP32 two-sum
+    pairs under compare-and-swap, FP32 two-sum through returning atomics, FP64
+    tile sums) against the FP64 oracle: e_max, e_rms <= 1e-6 per plane.  The
+    records are dense (3000 on grids of 40-70 pixels), so pixels collect
+    hundreds of contributions and lanes collide on pixels."""
+    from hpg_amd import workloads
+    monkeypatch.setenv("HPGX_FP32_TILE", scheme)
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[case]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=60 + case)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (0.9, 1.1), M, cf, 1, Mc)
+    rec = workloads.random_visibilities(3000, npol, gsz, (0.9, 1.1), cf, seed=70 + case,
+                                        phase=True)
+    print(f"scheme {scheme} case {case}", flush=True)
+    gs.grid_visibilities(hpg.Device.Serial, rec, True)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    got, exp = gs.grid_values(), o.grid_values()
+    for c in range(gsz[3]):
+        for r in range(gsz[2]):
+            if np.abs(exp[c, r]).max() == 0:
+                continue
+            e_max, e_rms = rel_errors(got[c, r], exp[c, r])
+            assert e_max <= 1e-6 and e_rms <= 1e-6, (c, r, e_max, e_rms)
+
+
 @pytest.mark.parametrize("radius", [(15, 2), (2, 15), (16, 1), (0, 0)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_support_at_tile_edge(hpg, oracle_mod, radius, mode):
