@@ -354,6 +354,7 @@ void free_state(hpgx_state_t st) {
   dev_free(cb.off_table);
   dev_free(cb.halo);
   dev_free(cb.halo_set);
+  dev_free(cb.degrid_acc);
   dev_free(cb.scan_temp);
   dev_free(cb.invalid_count);
   if (st->plan_z2z.valid) hipfftDestroy(st->plan_z2z.h);
@@ -518,6 +519,45 @@ int upload_model(hpgx_state_t st, const void* values, int layout, int memory) {
 // gridding / degridding pipeline on records already in cb.vis
 // gridding / degridding pipeline on records already in cb.vis; the records
 // are gridded in chunks whose bin entries fit the entry buffer
+// binned degridding of n records (cb.vis, or `records` read in place):
+// predicted values into the records (residuals and gvis when do_grid), or
+// FP64 predictions into pred64.  Requires use_halo(st) and a model.
+int run_degrid_binned(hpgx_state_t st, size_t n, bool do_grid, const void* records,
+                      double2* pred64) {
+  const Geometry geo = st->geometry();
+  CallBuffers& cb = st->cb;
+  const int np = int(st->npol);
+  const size_t acc_need = std::max<size_t>(st->max_batch, 1) * np;
+  if (acc_need > cb.degrid_acc_capacity) {
+    dev_free(cb.degrid_acc);
+    cb.degrid_acc_capacity = 0;
+    HPGX_HIP(st, dev_alloc(cb.degrid_acc, acc_need * sizeof(double2)));
+    cb.degrid_acc_capacity = acc_need;
+  }
+  HPGX_HIP(st, hipMemsetAsync(cb.degrid_acc, 0, n * np * sizeof(double2), st->stream));
+  const size_t epr = entries_per_record(st);
+  const size_t block_cap = hpgx::bin_block_records() * epr;
+  const size_t chunk = std::max<size_t>(1, st->pass_blocks) * hpgx::bin_block_records();
+  const size_t rec_bytes = visdata_size(st->npol);
+  const void* rec_base = records ? records : cb.vis;
+  const bool one_plane = st->gsize[2] == 1;
+  for (size_t off = 0; off < n; off += chunk) {
+    const size_t m = std::min(chunk, n - off);
+    CallBuffers c = cb;
+    c.vis = const_cast<char*>(static_cast<const char*>(rec_base)) + off * rec_bytes;
+    HPGX_HIP(st, hpgx::launch_bin(
+                   st->stream, np, m, geo, st->cf, st->mu, false, false,
+                   hpgx::GvisSource::Values, nullptr, c, block_cap, st->weights, true, true));
+    HPGX_HIP(st, hpgx::launch_degrid_tiles(
+                   st->stream, np, m, st->nkeys, geo, st->cf, st->mu, c, block_cap,
+                   halo_x(st), halo_y(st), st->model, cb.degrid_acc + off * np, one_plane));
+  }
+  HPGX_HIP(st, hpgx::launch_degrid_finish(
+                 st->stream, np, n, geo, st->cf, st->mu, cb.degrid_acc, do_grid, records, cb,
+                 pred64));
+  return HPGX_SUCCESS;
+}
+
 // records: device records read in place (nullptr: the staged copy cb.vis)
 int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
              bool do_grid, hpgx::GvisSource src, const double2* residuals,
@@ -528,8 +568,12 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
   const int np = int(st->npol);
   bool gvis_ready = false;
   if (do_degrid) {
-    HPGX_HIP(st, hpgx::launch_degrid(
-                   st->stream, np, n, geo, st->cf, st->mu, st->model, do_grid, cb));
+    if (st->model && use_halo(st)) {
+      if (int rc = run_degrid_binned(st, n, do_grid, nullptr, nullptr)) return rc;
+    } else {
+      HPGX_HIP(st, hpgx::launch_degrid(
+                     st->stream, np, n, geo, st->cf, st->mu, st->model, do_grid, cb));
+    }
     gvis_ready = true;
   }
   if (!do_grid) return HPGX_SUCCESS;
@@ -1081,9 +1125,13 @@ int hpgx_degrid_forward(
       && size_t(st->cf.max_sx) * size_t(st->cf.max_sy) <= hpgx::degrid_lane_max_taps();
     if (!in_place)
       if (int rc = stage_records(st, vis, n, memory)) return rc;
-    HPGX_HIP(st, hpgx::launch_degrid(st->stream, int(npol), n, st->geometry(), st->cf,
-                                     st->mu, st->model, false, st->cb, out,
-                                     in_place ? vis : nullptr));
+    if (st->model && use_halo(st)) {
+      if (int rc = run_degrid_binned(st, n, false, in_place ? vis : nullptr, out)) return rc;
+    } else {
+      HPGX_HIP(st, hpgx::launch_degrid(st->stream, int(npol), n, st->geometry(), st->cf,
+                                       st->mu, st->model, false, st->cb, out,
+                                       in_place ? vis : nullptr));
+    }
   }
   if (tmp) {
     HPGX_HIP(st, hipMemcpyAsync(predictions, tmp, bytes, hipMemcpyDeviceToHost, st->stream));

@@ -90,6 +90,8 @@ struct CallBuffers {
   size_t halo_capacity;  // elements of halo
   size_t halo_set_capacity;
   unsigned long long* invalid_count;
+  double2* degrid_acc;   // binned degridding: per record x polarization sums
+  size_t degrid_acc_capacity;
 };
 
 struct DeviceCF {
@@ -145,7 +147,7 @@ hipError_t launch_bin(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool update_weights, bool gvis_ready, GvisSource src,
   const double2* residuals, CallBuffers& cb, size_t block_cap, double* weights,
-  bool halo_entries);
+  bool halo_entries, bool degrid_pass = false);
 // haloed tiles (one entry per record and plane; supports below tile)
 size_t halo_entry_bytes(int npol);
 size_t halo_pixels_per_key(int hx, int hy);
@@ -161,6 +163,20 @@ hipError_t launch_tile_blocal(
   hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
   void* grid);
+// binned degridding: per-record sums over the footprint from the model's
+// LDS tiles (entries of launch_bin with degrid_pass), stored into acc when
+// each record has one plane, else added (acc zeroed beforehand)
+hipError_t launch_degrid_tiles(
+  hipStream_t s, int npol, size_t n, size_t nkeys, const Geometry& geo,
+  const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
+  int hx, int hy, const double2* model, double2* acc, bool one_plane);
+// predictions from those sums: into the records (cb.vis; residuals and gvis
+// when do_grid) or, with pred64, as FP64 values; records read from
+// `records` when given, else cb.vis
+hipError_t launch_degrid_finish(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, const double2* acc, bool do_grid, const void* records,
+  CallBuffers& cb, double2* pred64);
 // degridding; with pred64 the FP64 predictions are written there and the
 // records are only read, from `records` when given (else cb.vis; `records`
 // requires supports of at most degrid_lane_max_taps() taps)

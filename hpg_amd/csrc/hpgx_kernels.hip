@@ -27,6 +27,8 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <type_traits>
+
 namespace hpgx {
 namespace {
 
@@ -921,7 +923,7 @@ template <int N, bool HALO>
 __global__ __launch_bounds__(bin_threads) void k_bin(
   const VisRec<N>* __restrict__ recs, size_t n, Geometry geo,
   const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
-  Mueller mu, int update_weights, int gvis_ready, int adjoint,
+  Mueller mu, int update_weights, int gvis_ready, int adjoint, int degrid_pass,
   const float2* __restrict__ gvis_in, const double2* __restrict__ residuals,
   unsigned* __restrict__ off_table, void* __restrict__ entries_out,
   unsigned* __restrict__ entry_vis, size_t block_cap, double* __restrict__ weights,
@@ -972,7 +974,8 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
     bool valid = grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc);
     if (valid) valid = cfcube < unsigned(groups[grp].ncube);
     if (!valid) {
-      ++ninvalid;
+      // counted by the gridding pass only
+      if (!degrid_pass) ++ninvalid;
       continue;
     }
     const GroupInfo g = groups[grp];
@@ -990,7 +993,10 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
     e[j].h.fpk = unsigned(fxi) | (unsigned(fyi) << 8)
       | (unsigned((pos_w ? flag_pos_w : 0) | (grad ? flag_gradient : 0)) << 16);
     e[j].h.gpk = grp | (cfcube << 16);
-    if (gvis_ready) {
+    if (degrid_pass) {
+      // degridding reads the model: entries carry no values
+      for (int v = 0; v < N; ++v) e[j].g[v] = make_float2(0.0f, 0.0f);
+    } else if (gvis_ready) {
       for (int v = 0; v < N; ++v) e[j].g[v] = gvis_in[i * N + v];
     } else {
       const double2 phasor = cphase(r.phase);
@@ -1037,7 +1043,7 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
     const int tx0 = tiles_begin(c.gx), tx1 = HALO ? tx0 + 1 : tiles_end(c.gx, g.sx);
     const int ty0 = tiles_begin(c.gy), ty1 = HALO ? ty0 + 1 : tiles_end(c.gy, g.sy);
     for (int gpol = 0; gpol < geo.nr; ++gpol) {
-      if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
+      if (!row_has_index((pos_w != bool(degrid_pass)) ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
       const int base = (int(gcube) * geo.nr + gpol) * ntiles;
       for (int ty = ty0; ty < ty1; ++ty)
         for (int tx = tx0; tx < tx1; ++tx)
@@ -1091,7 +1097,7 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
     const int ty0 = tiles_begin(e[j].h.gy);
     const int ty1 = HALO ? ty0 + 1 : tiles_end(e[j].h.gy, sxy[j][1]);
     for (int gpol = 0; gpol < geo.nr; ++gpol) {
-      if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
+      if (!row_has_index((pos_w != bool(degrid_pass)) ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
       const int base = (int(gc[j]) * geo.nr + gpol) * ntiles;
       for (int ty = ty0; ty < ty1; ++ty)
         for (int tx = tx0; tx < tx1; ++tx) {
@@ -1108,7 +1114,7 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
           } else {
             static_cast<Entry<N>*>(entries_out)[b * block_cap + pos] = e[j];
           }
-          if (grad) outv[pos] = unsigned(b * bin_records + i);
+          if (grad || degrid_pass) outv[pos] = unsigned(b * bin_records + i);
         }
     }
   }
@@ -1351,7 +1357,9 @@ __global__ __launch_bounds__(block) void k_tile3(
 // order.  Requires hx, hy < tile.
 
 constexpr int halo_block = 256;
-constexpr int halo_chunk = 8;  // CF taps loaded together per lane
+constexpr int halo_chunk = 8;     // CF taps loaded together per lane
+constexpr int halo_classes = 32;  // LDS bank classes of the entry schedule
+constexpr int halo_sched = 2048;  // entries per scheduling round
 
 // pixels of the halo of one key: columns [tile, tile + hx) of rows
 // [0, tile), then all columns of rows [tile, tile + hy)
@@ -1376,21 +1384,33 @@ __device__ __forceinline__ unsigned long long pack_pair(float hi, float lo) {
     | ((unsigned long long)__float_as_uint(lo) << 32);
 }
 
-// adds h (+ the low-order part l) into the packed pair at a, starting from
-// the expected value `expect`; loops only while other lanes or waves change
-// the pair between the read and the swap, each retry following one of their
-// successful updates
+// the pair (hi, lo) after adding h (+ the low-order part l, when HAS_L)
+// to it: hi' = fl(hi + h), lo' = fl(lo + (err + l)) with err the exact
+// rounding error of the first sum (two-sum)
+template <bool HAS_L>
+__device__ __forceinline__ unsigned long long pair_plus(
+  unsigned long long pair, float h, float l) {
+  const float hi = __uint_as_float(unsigned(pair));
+  const float lo = __uint_as_float(unsigned(pair >> 32));
+  float sum, err;
+  two_sum(hi, h, sum, err);
+  return pack_pair(sum, HAS_L ? lo + (err + l) : lo + err);
+}
+
+// compare-and-swap of the pair at a from `expect` to `desired`; on failure
+// `expect` receives the value found
+__device__ __forceinline__ bool pair_swap(
+  unsigned long long* a, unsigned long long& expect, unsigned long long desired) {
+  return __hip_atomic_compare_exchange_strong(
+    a, &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// retries until the update of the pair at a lands; every retry follows a
+// successful update of the same pair by another lane or wave
+template <bool HAS_L>
 __device__ __forceinline__ void pair_add(
   unsigned long long* a, unsigned long long expect, float h, float l) {
-  for (;;) {
-    const float hi = __uint_as_float(unsigned(expect));
-    const float lo = __uint_as_float(unsigned(expect >> 32));
-    float sum, err;
-    two_sum(hi, h, sum, err);
-    const unsigned long long nw = pack_pair(sum, lo + (err + l));
-    if (__hip_atomic_compare_exchange_strong(a, &expect, nw, __ATOMIC_RELAXED,
-                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-      return;
+  while (!pair_swap(a, expect, pair_plus<HAS_L>(expect, h, l))) {
   }
 }
 
@@ -1415,6 +1435,8 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
   unsigned* seg_pref = reinterpret_cast<unsigned*>(smem + 2 * R);
   unsigned* seg_beg = seg_pref + nbins + 1;
   unsigned* part = seg_beg + nbins;
+  unsigned* cls = part + (halo_block / wave + 1);  // bank classes: counts, then cursors
+  unsigned* sched = cls + halo_classes;            // entry indexes of one round
 
   const unsigned key = blockIdx.x;
   const int nkeys = geo.ntx * geo.nty * geo.nr * geo.nc;
@@ -1446,74 +1468,135 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
   const int lane = threadIdx.x % wave;
   constexpr int nw = halo_block / wave;
 
-  for (unsigned base = w * wave; base < total; base += nw * wave) {
-    const unsigned q = base + lane;
-    if (q >= total) continue;
-    // locate entry q in the concatenated segments
+  // entry q of the key (0 <= q < total): its index in the entry array
+  auto locate = [&](unsigned q) {
     int lo = 0, hi = nbins;  // seg_pref[lo] <= q < seg_pref[hi]
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
       if (seg_pref[mid] <= q) lo = mid; else hi = mid;
     }
-    const unsigned idx = unsigned(lo) * unsigned(block_cap) + seg_beg[lo] + (q - seg_pref[lo]);
-    const Entry4<N> e = entries[idx];
-    const unsigned pos = e.h.pos;
-    const int lx = int(pos & 63u);
-    const int ly = int((pos >> 6) & 63u);
-    const int fxi = int((pos >> 12) & 255u);
-    const int fyi = int((pos >> 20) & 255u);
-    const bool pos_w = pos & e4_pos_w;
-    const bool grad = pos & e4_gradient;
-    const int grp = int(e.h.gpk & 0xffffu);
-    const int cfcube = int(e.h.gpk >> 16);
-    const GroupInfo g = groups[grp];
-    const int sx = g.sx, sy = g.sy;
-    const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
-    const float imf = pos_w ? -1.0f : 1.0f;
-    const float2* slice[N];
-    bool use[N];
-    const long long slice_taps = (long long)sx * sy;
-    for (int v = 0; v < N; ++v) {
-      const int m = row[v];
-      use[v] = m >= 0 && m < g.nmueller;
-      slice[v] = taps + g.tap_offset
-        + (((long long)(cfcube * g.nmueller + (use[v] ? m : 0)) * geo.nf + fxi) * geo.nf
-           + fyi) * slice_taps;
+    return unsigned(lo) * unsigned(block_cap) + seg_beg[lo] + (q - seg_pref[lo]);
+  };
+  // LDS bank class of an entry's first tap; every tap of every lane adds
+  // the same offset to it, so a wave whose lanes hold consecutive classes
+  // spreads each of its LDS updates evenly over the banks
+  auto bank_class = [&](unsigned pos) {
+    return unsigned(int((pos >> 6) & 63u) * RX + int(pos & 63u)) % unsigned(halo_classes);
+  };
+  // rank of this lane in a wave's sample of the class-sorted round: lanes
+  // l and l + 32 take neighbouring samples, so lanes 0-31 (and 32-63) hold
+  // ascending classes
+  const unsigned rank = unsigned((lane & 31) * 2 + (lane >> 5));
+
+  for (unsigned r0 = 0; r0 < total; r0 += halo_sched) {
+    const unsigned rn = min(unsigned(halo_sched), total - r0);
+    // counting sort of the round's entries by bank class
+    if (threadIdx.x < halo_classes) cls[threadIdx.x] = 0u;
+    __syncthreads();
+    for (unsigned q = threadIdx.x; q < rn; q += halo_block)
+      __hip_atomic_fetch_add(cls + bank_class(entries[locate(r0 + q)].h.pos), 1u,
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned run = 0;
+      for (int c = 0; c < halo_classes; ++c) {
+        const unsigned t = cls[c];
+        cls[c] = run;
+        run += t;
+      }
     }
-    double phi0x = 0.0, phi0y = 0.0, dphix = 0.0, dphiy = 0.0;
-    if (grad) {
-      const unsigned i = entry_vis[idx];
-      const double gxr = recs[i].grad[0];
-      const double gyr = recs[i].grad[1];
-      phi0x = -gxr * ((sx / 2) * geo.os - (fxi - geo.F));
-      phi0y = -gyr * ((sy / 2) * geo.os - (fyi - geo.F));
-      dphix = gxr * geo.os;
-      dphiy = gyr * geo.os;
+    __syncthreads();
+    for (unsigned q = threadIdx.x; q < rn; q += halo_block) {
+      const unsigned idx = locate(r0 + q);
+      const unsigned slot = __hip_atomic_fetch_add(
+        cls + bank_class(entries[idx].h.pos), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      sched[slot] = idx;
     }
-    for (int Y = 0; Y < sy; ++Y) {
-      const int lrow = (ly + Y) * RX + lx;
-      for (int X0 = 0; X0 < sx; X0 += halo_chunk) {
+    __syncthreads();
+    // batch j of the round takes sorted entries rank * nbatch + j
+    const unsigned nbatch = (rn + wave - 1) / wave;
+    for (unsigned j = w; j < nbatch; j += nw) {
+      const unsigned sp = rank * nbatch + j;
+      if (sp >= rn) continue;
+      const unsigned idx = sched[sp];
+      const Entry4<N> e = entries[idx];
+      const unsigned pos = e.h.pos;
+      const int lx = int(pos & 63u);
+      const int ly = int((pos >> 6) & 63u);
+      const int fxi = int((pos >> 12) & 255u);
+      const int fyi = int((pos >> 20) & 255u);
+      const bool pos_w = pos & e4_pos_w;
+      const bool grad = pos & e4_gradient;
+      const int grp = int(e.h.gpk & 0xffffu);
+      const int cfcube = int(e.h.gpk >> 16);
+      const GroupInfo g = groups[grp];
+      const int sx = g.sx, sy = g.sy;
+      const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
+      const float imf = pos_w ? -1.0f : 1.0f;
+      const float2* slice[N];
+      bool use[N];
+      const long long slice_taps = (long long)sx * sy;
+      for (int v = 0; v < N; ++v) {
+        const int m = row[v];
+        use[v] = m >= 0 && m < g.nmueller;
+        slice[v] = taps + g.tap_offset
+          + (((long long)(cfcube * g.nmueller + (use[v] ? m : 0)) * geo.nf + fxi) * geo.nf
+             + fyi) * slice_taps;
+      }
+      double phi0x = 0.0, phi0y = 0.0, dphix = 0.0, dphiy = 0.0;
+      if (grad) {
+        const unsigned i = entry_vis[idx];
+        const double gxr = recs[i].grad[0];
+        const double gyr = recs[i].grad[1];
+        phi0x = -gxr * ((sx / 2) * geo.os - (fxi - geo.F));
+        phi0y = -gyr * ((sy / 2) * geo.os - (fyi - geo.F));
+        dphix = gxr * geo.os;
+        dphiy = gyr * geo.os;
+      }
+      // the footprint's taps in slice order (t = Y * sx + X), halo_chunk at a
+      // time; full chunks are straight-line code, so each chunk's loads and
+      // LDS updates issue back to back
+      const int ntap = sx * sy;
+      int cx = 0, cy = 0;  // (X, Y) of the chunk's first tap
+      auto do_chunk = [&](auto full, int t0) {
+        constexpr bool FULL = decltype(full)::value;
+        bool ok[halo_chunk];
+        int Xk[halo_chunk], Yk[halo_chunk], lik[halo_chunk];
+  #pragma unroll
+        for (int k = 0; k < halo_chunk; ++k) {
+          ok[k] = FULL || t0 + k < ntap;
+          Xk[k] = cx;
+          Yk[k] = cy;
+          lik[k] = (ly + cy) * RX + lx + cx;
+          if (++cx == sx) {
+            cx = 0;
+            ++cy;
+          }
+        }
         float2 c[N][halo_chunk];
-#pragma unroll
+  #pragma unroll
         for (int k = 0; k < halo_chunk; ++k)
-#pragma unroll
-          for (int v = 0; v < N; ++v)
-            c[v][k] = (X0 + k < sx && use[v]) ? slice[v][Y * sx + X0 + k]
-                                              : make_float2(0.0f, 0.0f);
+  #pragma unroll
+          for (int v = 0; v < N; ++v) {
+            // slice[v] is a valid slice also when !use[v] (Mueller index 0)
+            float2 tv = make_float2(0.0f, 0.0f);
+            if (ok[k]) tv = slice[v][t0 + k];
+            c[v][k] = use[v] ? tv : make_float2(0.0f, 0.0f);
+          }
         // products of the chunk's taps: hi parts (FP32) and, for N > 1, the
         // low-order parts of their FP64 sums over the polarizations
         float hr[halo_chunk], him[halo_chunk], lr[halo_chunk], lim[halo_chunk];
         double dr[halo_chunk], di[halo_chunk];
-#pragma unroll
+  #pragma unroll
         for (int k = 0; k < halo_chunk; ++k) {
-          const int X = X0 + k;
           float2 scr = make_float2(1.0f, 0.0f);
-          if (grad && X < sx) scr = narrow(cphase((phi0x + X * dphix) + (phi0y + Y * dphiy)));
+          if (grad && ok[k])
+            scr = narrow(cphase((phi0x + Xk[k] * dphix) + (phi0y + Yk[k] * dphiy)));
           if constexpr (N == 1) {
             float2 cc = c[0][k];
             cc.y *= imf;
             if (grad) cc = cmul(cc, scr);
-            const float2 pr = use[0] ? cmul(cc, e.g[0]) : make_float2(0.0f, 0.0f);
+            const float2 pr = cmul(cc, e.g[0]);
             hr[k] = pr.x;
             him[k] = pr.y;
             lr[k] = 0.0f;
@@ -1522,7 +1605,7 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
             di[k] = double(pr.y);
           } else {
             double sr = 0.0, si = 0.0;
-#pragma unroll
+  #pragma unroll
             for (int v = 0; v < N; ++v) {
               if (!use[v]) continue;
               float2 cc = c[v][k];
@@ -1541,20 +1624,19 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
           }
         }
         if constexpr (ACC == acc_f64) {
-#pragma unroll
+  #pragma unroll
           for (int k = 0; k < halo_chunk; ++k) {
-            if (X0 + k >= sx) break;
-            const int li = lrow + X0 + k;
-            __hip_atomic_fetch_add(acc64 + li, dr[k], __ATOMIC_RELAXED,
+            if (!ok[k]) continue;
+            __hip_atomic_fetch_add(acc64 + lik[k], dr[k], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(acc64 + R + li, di[k], __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(acc64 + R + lik[k], di[k], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         } else if constexpr (ACC == acc_f32_rtn) {
-#pragma unroll
+  #pragma unroll
           for (int k = 0; k < halo_chunk; ++k) {
-            if (X0 + k >= sx) break;
-            const int li = lrow + X0 + k;
+            if (!ok[k]) continue;
+            const int li = lik[k];
             const float orr = __hip_atomic_fetch_add(
               acc32 + li, hr[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             const float ori = __hip_atomic_fetch_add(
@@ -1568,28 +1650,35 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         } else {
-          // read the chunk's pairs first, then swap each from that value
-          unsigned long long er[halo_chunk], ei[halo_chunk];
+          // pixel li holds (H_re, L_re) at pairs[li] and (H_im, L_im) at
+        // pairs[R + li].  Read the chunk's pixels, then swap each from the
+        // value read, settling each swap before the next (issuing all
+        // swaps first measured slower: the longer window between read and
+        // swap makes more of them fail).  A read may see a pair mid-update;
+        // its swap then fails and the retry starts from the value the swap
+        // returned.
+        constexpr bool has_l = N > 1;
+        unsigned long long er[halo_chunk], ei[halo_chunk];
 #pragma unroll
-          for (int k = 0; k < halo_chunk; ++k) {
-            if (X0 + k >= sx) break;
-            const int li = lrow + X0 + k;
-            er[k] = __hip_atomic_load(pairs + li, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            ei[k] = __hip_atomic_load(pairs + R + li, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-          }
+        for (int k = 0; k < halo_chunk; ++k) {
+          if (!ok[k]) continue;
+          er[k] = pairs[lik[k]];
+          ei[k] = pairs[R + lik[k]];
+        }
 #pragma unroll
-          for (int k = 0; k < halo_chunk; ++k) {
-            if (X0 + k >= sx) break;
-            const int li = lrow + X0 + k;
-            pair_add(pairs + li, er[k], hr[k], lr[k]);
-            pair_add(pairs + R + li, ei[k], him[k], lim[k]);
-          }
+        for (int k = 0; k < halo_chunk; ++k) {
+          if (!ok[k]) continue;
+          pair_add<has_l>(pairs + lik[k], er[k], hr[k], lr[k]);
+          pair_add<has_l>(pairs + R + lik[k], ei[k], him[k], lim[k]);
         }
       }
+    };
+      int t0 = 0;
+      for (; t0 + halo_chunk <= ntap; t0 += halo_chunk) do_chunk(std::true_type{}, t0);
+      if (t0 < ntap) do_chunk(std::false_type{}, t0);
     }
+    __syncthreads();  // the next round reuses cls and sched
   }
-  __syncthreads();
 
   const int hs = halo_pixels(hx, hy);
   double2* hk = halo + size_t(key) * hs;
@@ -1942,6 +2031,225 @@ __global__ __launch_bounds__(block) void k_degrid_lane(
   }
 }
 
+// Binned degridding (supports below a tile): the entries of k_bin<N, true>
+// in degrid mode (record index beside every entry) are processed one
+// workgroup per (plane, tile) key.  The workgroup loads the model's tile +
+// halo region of its plane into LDS with coalesced reads, then one lane per
+// entry sums c' x model over the footprint from LDS (FP64), in slice-order
+// chunks whose CF loads issue together.  The per-plane sums go to acc
+// (stored when one plane per record, else added atomically); k_degrid_finish
+// turns them into predictions.
+template <int N>
+__global__ __launch_bounds__(halo_block) void k_degrid4(
+  Geometry geo, const GroupInfo* __restrict__ groups,
+  const float2* __restrict__ taps, Mueller mu, const unsigned* __restrict__ off_table,
+  int nbins, size_t block_cap, const Entry4<N>* __restrict__ entries,
+  const unsigned* __restrict__ entry_vis, const VisRec<N>* __restrict__ recs,
+  int hx, int hy, const double2* __restrict__ model, double2* __restrict__ acc,
+  int one_plane) {
+
+  extern __shared__ double smem[];
+  const int RX = tile + hx;
+  const int RY = tile + hy;
+  const int R = RX * RY;
+  double2* mreg = reinterpret_cast<double2*>(smem);
+  unsigned* seg_pref = reinterpret_cast<unsigned*>(smem + 2 * R);
+  unsigned* seg_beg = seg_pref + nbins + 1;
+  unsigned* part = seg_beg + nbins;
+
+  const unsigned key = blockIdx.x;
+  const int nkeys = geo.ntx * geo.nty * geo.nr * geo.nc;
+  for (int bb = threadIdx.x; bb < nbins; bb += halo_block) {
+    const unsigned* tab = off_table + size_t(bb) * (nkeys + 1);
+    const unsigned s0 = tab[key], s1 = tab[key + 1];
+    seg_pref[bb] = s1 - s0;
+    seg_beg[bb] = s0;
+  }
+  __syncthreads();
+  const unsigned total = block_exclusive_scan(seg_pref, nbins, part);
+  if (total == 0) return;
+  if (threadIdx.x == 0) seg_pref[nbins] = total;
+
+  const int ntiles = geo.ntx * geo.nty;
+  const unsigned plane = key / ntiles;
+  const int t = int(key % ntiles);
+  const int x0 = (t % geo.ntx) * tile;
+  const int y0 = (t / geo.ntx) * tile;
+  const int gpol = int(plane % geo.nr);
+  const double2* mplane = model + size_t(plane) * geo.ny * size_t(geo.nx);
+  for (int k = threadIdx.x; k < R; k += halo_block) {
+    const int ry = k / RX;
+    const int rx = k - ry * RX;
+    const int xx = x0 + rx, yy = y0 + ry;
+    mreg[k] = (xx < geo.nx && yy < geo.ny) ? mplane[size_t(yy) * geo.nx + xx]
+                                           : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+
+  const int w = threadIdx.x / wave;
+  const int lane = threadIdx.x % wave;
+  constexpr int nw = halo_block / wave;
+  for (unsigned base = w * wave; base < total; base += nw * wave) {
+    const unsigned q = base + lane;
+    if (q >= total) continue;
+    int lo = 0, hi = nbins;  // seg_pref[lo] <= q < seg_pref[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (seg_pref[mid] <= q) lo = mid; else hi = mid;
+    }
+    const unsigned idx = unsigned(lo) * unsigned(block_cap) + seg_beg[lo] + (q - seg_pref[lo]);
+    const Entry4Head eh = entries[idx].h;
+    const unsigned i = entry_vis[idx];
+    const unsigned pos = eh.pos;
+    const int lx = int(pos & 63u);
+    const int ly = int((pos >> 6) & 63u);
+    const int fxi = int((pos >> 12) & 255u);
+    const int fyi = int((pos >> 20) & 255u);
+    const bool pos_w = pos & e4_pos_w;
+    const bool grad = pos & e4_gradient;
+    const int grp = int(eh.gpk & 0xffffu);
+    const int cfcube = int(eh.gpk >> 16);
+    const GroupInfo g = groups[grp];
+    const int sx = g.sx, sy = g.sy;
+    // degridding uses the other Mueller matrix and the opposite conjugation
+    // of gridding (src/hpg_core.hpp:598-600)
+    const int* row = pos_w ? mu.Mc[gpol] : mu.M[gpol];
+    const float imf = pos_w ? 1.0f : -1.0f;
+    const float2* slice[N];
+    bool use[N];
+    const long long slice_taps = (long long)sx * sy;
+    for (int v = 0; v < N; ++v) {
+      const int m = row[v];
+      use[v] = m >= 0 && m < g.nmueller;
+      slice[v] = taps + g.tap_offset
+        + (((long long)(cfcube * g.nmueller + (use[v] ? m : 0)) * geo.nf + fxi) * geo.nf
+           + fyi) * slice_taps;
+    }
+    double phi0x = 0.0, phi0y = 0.0, dphix = 0.0, dphiy = 0.0;
+    if (grad) {
+      const double gxr = recs[i].grad[0];
+      const double gyr = recs[i].grad[1];
+      phi0x = -gxr * ((sx / 2) * geo.os - (fxi - geo.F));
+      phi0y = -gyr * ((sy / 2) * geo.os - (fyi - geo.F));
+      dphix = gxr * geo.os;
+      dphiy = gyr * geo.os;
+    }
+    double2 a[N];
+    for (int v = 0; v < N; ++v) a[v] = make_double2(0.0, 0.0);
+    const int ntap = sx * sy;
+    int cx = 0, cy = 0;
+    auto do_chunk = [&](auto full, int t0) {
+      constexpr bool FULL = decltype(full)::value;
+      bool ok[halo_chunk];
+      int Xk[halo_chunk], Yk[halo_chunk];
+      double2 mv[halo_chunk];
+#pragma unroll
+      for (int k = 0; k < halo_chunk; ++k) {
+        ok[k] = FULL || t0 + k < ntap;
+        Xk[k] = cx;
+        Yk[k] = cy;
+        mv[k] = ok[k] ? mreg[(ly + cy) * RX + lx + cx] : make_double2(0.0, 0.0);
+        if (++cx == sx) {
+          cx = 0;
+          ++cy;
+        }
+      }
+      float2 c[N][halo_chunk];
+#pragma unroll
+      for (int k = 0; k < halo_chunk; ++k)
+#pragma unroll
+        for (int v = 0; v < N; ++v) {
+          float2 tv = make_float2(0.0f, 0.0f);
+          if (ok[k]) tv = slice[v][t0 + k];
+          c[v][k] = tv;
+        }
+#pragma unroll
+      for (int k = 0; k < halo_chunk; ++k) {
+        if (!ok[k]) continue;
+        double2 m = mv[k];
+        if (grad) {
+          const double phX = phi0x + Xk[k] * dphix;
+          const double phY = phi0y + Yk[k] * dphiy;
+          m = cmul(m, cphase(-phX - phY));
+        }
+#pragma unroll
+        for (int v = 0; v < N; ++v) {
+          if (!use[v]) continue;
+          float2 cc = c[v][k];
+          cc.y *= imf;
+          const double2 p = cmul(cc, m);
+          a[v].x += p.x;
+          a[v].y += p.y;
+        }
+      }
+    };
+    int t0 = 0;
+    for (; t0 + halo_chunk <= ntap; t0 += halo_chunk) do_chunk(std::true_type{}, t0);
+    if (t0 < ntap) do_chunk(std::false_type{}, t0);
+    for (int v = 0; v < N; ++v) {
+      double2* dst = acc + size_t(i) * N + v;
+      if (one_plane) {
+        *dst = a[v];
+      } else {
+        atomicAdd(&dst->x, a[v].x);
+        atomicAdd(&dst->y, a[v].y);
+      }
+    }
+  }
+}
+
+// predictions from the per-record sums of k_degrid4 (the end of degrid_vis
+// and of the degridding branch of VisibilityGridder::kernel,
+// src/hpg_core.hpp:643-682, 1136-1169), one thread per record
+template <int N>
+__global__ __launch_bounds__(block) void k_degrid_finish(
+  size_t n, Geometry geo, const GroupInfo* __restrict__ groups,
+  const double2* __restrict__ wsum, Mueller mu, const double2* __restrict__ acc,
+  int do_grid, const VisRec<N>* recs_in, VisRec<N>* recs, float2* __restrict__ gvis,
+  double2* __restrict__ pred64) {
+  // recs_in and recs may be the same records: each thread reads its
+  // record's fields before writing them back
+  const size_t i = blockIdx.x * (size_t)block + threadIdx.x;
+  if (i >= n) return;
+  const VisRec<N>& r = recs_in[i];
+  const unsigned grp = r.cf_index[1];
+  const unsigned cfcube = r.cf_index[0];
+  const unsigned gcube = r.grid_cube;
+  if (!(grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc))) return;
+  const GroupInfo g = groups[grp];
+  if (!(cfcube < unsigned(g.ncube))) return;
+  const Coord c = vis_coord(r, geo, g);
+  if (!c.inside) return;
+  const bool pos_w = r.uvw[2] > 0;
+  const double2 phasor = cphase(r.phase);
+  double2 W[N];
+  degrid_norms<N>(geo, g, wsum, mu, pos_w, cfcube, c.fx + geo.F, c.fy + geo.F, W);
+  const double2 conj_phasor = make_double2(phasor.x, -phasor.y);
+  float2 pred[N];
+  double2 pred_d[N];
+  for (int v = 0; v < N; ++v) {
+    const double2 d = (W[v].x != 0.0 || W[v].y != 0.0) ? W[v] : make_double2(1.0, 0.0);
+    pred_d[v] = cmul(kdiv(acc[i * N + v], d), conj_phasor);
+    pred[v] = narrow(pred_d[v]);
+  }
+  if (pred64) {
+    for (int v = 0; v < N; ++v) pred64[i * N + v] = pred_d[v];
+    return;
+  }
+  VisRec<N>& o = recs[i];
+  for (int v = 0; v < N; ++v) {
+    if (do_grid) {
+      float2 val = r.vis[v];
+      val.x -= pred[v].x;
+      val.y -= pred[v].y;
+      o.vis[v] = val;
+      gvis[i * N + v] = prephase(val, phasor, r.w[v]);
+    } else {
+      o.vis[v] = pred[v];
+    }
+  }
+}
+
 // ---- finishing passes ------------------------------------------------------
 
 template <typename T>
@@ -2221,15 +2529,16 @@ hipError_t launch_bin(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool update_weights, bool gvis_ready, GvisSource src,
   const double2* residuals, CallBuffers& cb, size_t block_cap, double* weights,
-  bool halo_entries) {
+  bool halo_entries, bool degrid_pass) {
   if (n == 0) return hipSuccess;
   const int adj = src == GvisSource::Adjoint;
+  const int dg = degrid_pass;
   const unsigned nb = unsigned(bin_blocks_for(n));
 #define HPGX_BIN(NP, H)                                                        \
   hipLaunchKernelGGL(                                                          \
     (k_bin<NP, H>), dim3(nb), dim3(bin_threads), 0, s,                         \
     static_cast<const VisRec<NP>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,    \
-    int(update_weights), int(gvis_ready), adj, cb.gvis, residuals,             \
+    int(update_weights), int(gvis_ready), adj, dg, cb.gvis, residuals,         \
     cb.off_table, cb.stream_entries, cb.entry_vis, block_cap, weights,         \
     cb.invalid_count)
   if (halo_entries) {
@@ -2260,7 +2569,9 @@ size_t halo_pixels_per_key(int hx, int hy) { return size_t(halo_pixels(hx, hy));
 int halo_tile_limit() { return tile; }
 size_t halo_lds_bytes(int hx, int hy, int nbins) {
   const size_t r = size_t(tile + hx) * (tile + hy);
-  return r * 16 + sizeof(unsigned) * (2 * size_t(nbins) + 1 + halo_block / wave + 1);
+  return r * 16
+    + sizeof(unsigned)
+      * (2 * size_t(nbins) + 1 + halo_block / wave + 1 + halo_classes + halo_sched);
 }
 
 hipError_t launch_tile_halo(
@@ -2347,6 +2658,55 @@ hipError_t launch_tile_blocal(
 }
 
 size_t degrid_lane_max_taps() { return size_t(degrid_lane_taps); }
+
+hipError_t launch_degrid_tiles(
+  hipStream_t s, int npol, size_t n, size_t nkeys, const Geometry& geo,
+  const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
+  int hx, int hy, const double2* model, double2* acc, bool one_plane) {
+  if (nkeys == 0 || n == 0) return hipSuccess;
+  if (hx < 0 || hy < 0 || hx >= tile || hy >= tile) return hipErrorInvalidValue;
+  const int nb = bin_blocks_for(n);
+  const size_t lds = halo_lds_bytes(hx, hy, nb);
+  if (lds > 65536) return hipErrorInvalidValue;
+#define HPGX_DEGRID4(NP)                                                       \
+  hipLaunchKernelGGL(                                                          \
+    k_degrid4<NP>, dim3(unsigned(nkeys)), dim3(halo_block), lds, s, geo,       \
+    cf.groups, cf.taps, mu, cb.off_table, nb, block_cap,                       \
+    static_cast<const Entry4<NP>*>(cb.stream_entries), cb.entry_vis,           \
+    static_cast<const VisRec<NP>*>(cb.vis), hx, hy, model, acc, int(one_plane))
+  switch (npol) {
+  case 1: HPGX_DEGRID4(1); break;
+  case 2: HPGX_DEGRID4(2); break;
+  case 3: HPGX_DEGRID4(3); break;
+  case 4: HPGX_DEGRID4(4); break;
+  default: return hipErrorInvalidValue;
+  }
+#undef HPGX_DEGRID4
+  return hipGetLastError();
+}
+
+hipError_t launch_degrid_finish(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, const double2* acc, bool do_grid, const void* records,
+  CallBuffers& cb, double2* pred64) {
+  if (n == 0) return hipSuccess;
+  const void* in = records ? records : cb.vis;
+#define HPGX_DFIN(NP)                                                          \
+  hipLaunchKernelGGL(                                                          \
+    k_degrid_finish<NP>, dim3(blocks_for(n)), dim3(block), 0, s, n, geo,       \
+    cf.groups, cf.wsum, mu, acc, int(do_grid),                                 \
+    static_cast<const VisRec<NP>*>(in), static_cast<VisRec<NP>*>(cb.vis),      \
+    cb.gvis, pred64)
+  switch (npol) {
+  case 1: HPGX_DFIN(1); break;
+  case 2: HPGX_DFIN(2); break;
+  case 3: HPGX_DFIN(3); break;
+  case 4: HPGX_DFIN(4); break;
+  default: return hipErrorInvalidValue;
+  }
+#undef HPGX_DFIN
+  return hipGetLastError();
+}
 
 hipError_t launch_degrid(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,

@@ -150,22 +150,35 @@ def test_adjoint_is_gridding_of_scaled_residuals(hpg):
     assert np.abs(got - exp).max() <= 1e-6 * np.abs(exp).max()
 
 
-def test_chunked_gridding_equals_one_pass(hpg, monkeypatch):
-    """A batch whose bin entries exceed the entry budget is gridded in
-    chunks with the same result."""
+@pytest.mark.parametrize("case", [2, 3])
+def test_chunked_gridding_equals_one_pass(hpg, monkeypatch, case):
+    """A batch whose bin entries exceed the entry budget is gridded (and
+    degridded) in chunks with the same result: case 2 runs the haloed tiles
+    and the binned degridder (two grid rows), case 3 (35-pixel support) the
+    clipped tiles."""
     from hpg_amd import workloads
     from test_gpu_parity import CASES
-    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[3]
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[case]
     cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=50)
-    rec = workloads.random_visibilities(20000, npol, gsz, (1.0, 1.0), cf, seed=51)
-    gs1 = hpg.GridderState.create(hpg.Device.Hip, 0, 20000, cf, gsz, (1.0, 1.0), M, Mc)
-    gs1.set_convolution_function(hpg.Device.Serial, cf)
-    gs1.grid_visibilities(hpg.Device.Serial, rec)
-    monkeypatch.setenv("HPGX_ENTRY_BUDGET_MB", "1")
-    gs2 = hpg.GridderState.create(hpg.Device.Hip, 0, 20000, cf, gsz, (1.0, 1.0), M, Mc)
-    gs2.set_convolution_function(hpg.Device.Serial, cf)
-    gs2.grid_visibilities(hpg.Device.Serial, rec)
+    rec = workloads.random_visibilities(20000, npol, gsz, (1.0, 1.0), cf, seed=51,
+                                        phase=True)
+    rng = np.random.default_rng(52)
+    shape = (gsz[3], gsz[2], gsz[1], gsz[0])
+    model = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    states = []
+    for budget in (None, "1"):
+        if budget:
+            monkeypatch.setenv("HPGX_ENTRY_BUDGET_MB", budget)
+        gs = hpg.GridderState.create(hpg.Device.Hip, 0, 20000, cf, gsz, (1.0, 1.0), M, Mc)
+        gs.set_convolution_function(hpg.Device.Serial, cf)
+        gs.grid_visibilities(hpg.Device.Serial, rec)
+        gs.set_model(hpg.Device.Serial, model)
+        pred = gs.degrid_forward(rec)
+        states.append((gs, pred))
+    (gs1, p1), (gs2, p2) = states
     a, b = gs1.grid_values(), gs2.grid_values()
     assert np.abs(a).max() > 0
     assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
     np.testing.assert_allclose(gs2.grid_weights(), gs1.grid_weights(), rtol=1e-12)
+    assert np.abs(p1).max() > 0
+    assert np.abs(p1 - p2).max() <= 1e-12 * np.abs(p1).max()

@@ -239,12 +239,25 @@ def test_serialization_and_batches(hpg, oracle_mod):
     np.testing.assert_allclose(gs1.grid_weights(), gs2.grid_weights(), rtol=1e-13)
 
 
-def test_degrid_and_residual_vs_oracle(hpg, oracle_mod):
+@pytest.mark.parametrize("gridder", [3, 4])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_degrid_and_residual_vs_oracle(hpg, oracle_mod, monkeypatch, gridder, case):
+    """Predicted values, residuals and the gridded residuals against the
+    oracle: 1e-6 relative (FP32 outputs of FP64 sums), grids to 1e-5.  With
+    the haloed-tile gridder (4) and supports below 32 pixels the degridder
+    is binned (model tiles in LDS, per-record sums, then the finishing pass,
+    summed atomically over grid rows when there are several); otherwise it
+    runs one lane per record (3; and case 3, whose support is 35 pixels, one
+    wave per record).  Phase gradients on half the records, records off the
+    grid, both signs of w."""
     from hpg_amd import workloads
-    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[3]
-    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=7)
-    rec = workloads.random_visibilities(800, npol, gsz, (1.0, 1.0), cf, seed=8,
-                                        gradient=True, phase=True)
+    monkeypatch.setenv("HPGX_GRIDDER", str(gridder))
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[case]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=7 + case)
+    rec = workloads.random_visibilities(800, npol, gsz, (1.0, 1.0), cf, seed=8 + case,
+                                        gradient=True, phase=True, margin=-2)
+    rec["cf_phase_gradient"][::2] = 0.0
+    print(f"degrid case {case} gridder {gridder}", flush=True)
     rng = np.random.default_rng(9)
     shape = (gsz[3], gsz[2], gsz[1], gsz[0])
     model = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
