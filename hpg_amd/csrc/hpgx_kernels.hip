@@ -1553,129 +1553,140 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
         dphix = gxr * geo.os;
         dphiy = gyr * geo.os;
       }
-      // the footprint's taps in slice order (t = Y * sx + X), halo_chunk at a
-      // time; full chunks are straight-line code, so each chunk's loads and
-      // LDS updates issue back to back
-      const int ntap = sx * sy;
-      int cx = 0, cy = 0;  // (X, Y) of the chunk's first tap
-      auto do_chunk = [&](auto full, int t0) {
-        constexpr bool FULL = decltype(full)::value;
-        bool ok[halo_chunk];
-        int Xk[halo_chunk], Yk[halo_chunk], lik[halo_chunk];
-  #pragma unroll
-        for (int k = 0; k < halo_chunk; ++k) {
-          ok[k] = FULL || t0 + k < ntap;
-          Xk[k] = cx;
-          Yk[k] = cy;
-          lik[k] = (ly + cy) * RX + lx + cx;
-          if (++cx == sx) {
-            cx = 0;
-            ++cy;
-          }
-        }
-        float2 c[N][halo_chunk];
-  #pragma unroll
-        for (int k = 0; k < halo_chunk; ++k)
-  #pragma unroll
+      // the value of one tap: hi (FP32) and, for N > 1, the low-order part
+      // of the FP64 sum over the polarizations; d = the FP64 value
+      auto tap_value = [&](const float2* cv, int X, int Y, float& h_r, float& h_i,
+                           float& l_r, float& l_i, double& d_r, double& d_i) {
+        float2 scr = make_float2(1.0f, 0.0f);
+        if (grad) scr = narrow(cphase((phi0x + X * dphix) + (phi0y + Y * dphiy)));
+        if constexpr (N == 1) {
+          float2 cc = cv[0];
+          cc.y *= imf;
+          if (grad) cc = cmul(cc, scr);
+          const float2 pr = cmul(cc, e.g[0]);
+          h_r = pr.x;
+          h_i = pr.y;
+          l_r = 0.0f;
+          l_i = 0.0f;
+          d_r = double(pr.x);
+          d_i = double(pr.y);
+        } else {
+          double sr = 0.0, si = 0.0;
+#pragma unroll
           for (int v = 0; v < N; ++v) {
-            // slice[v] is a valid slice also when !use[v] (Mueller index 0)
-            float2 tv = make_float2(0.0f, 0.0f);
-            if (ok[k]) tv = slice[v][t0 + k];
-            c[v][k] = use[v] ? tv : make_float2(0.0f, 0.0f);
-          }
-        // products of the chunk's taps: hi parts (FP32) and, for N > 1, the
-        // low-order parts of their FP64 sums over the polarizations
-        float hr[halo_chunk], him[halo_chunk], lr[halo_chunk], lim[halo_chunk];
-        double dr[halo_chunk], di[halo_chunk];
-  #pragma unroll
-        for (int k = 0; k < halo_chunk; ++k) {
-          float2 scr = make_float2(1.0f, 0.0f);
-          if (grad && ok[k])
-            scr = narrow(cphase((phi0x + Xk[k] * dphix) + (phi0y + Yk[k] * dphiy)));
-          if constexpr (N == 1) {
-            float2 cc = c[0][k];
+            if (!use[v]) continue;
+            float2 cc = cv[v];
             cc.y *= imf;
             if (grad) cc = cmul(cc, scr);
-            const float2 pr = cmul(cc, e.g[0]);
-            hr[k] = pr.x;
-            him[k] = pr.y;
-            lr[k] = 0.0f;
-            lim[k] = 0.0f;
-            dr[k] = double(pr.x);
-            di[k] = double(pr.y);
-          } else {
-            double sr = 0.0, si = 0.0;
-  #pragma unroll
-            for (int v = 0; v < N; ++v) {
-              if (!use[v]) continue;
-              float2 cc = c[v][k];
-              cc.y *= imf;
-              if (grad) cc = cmul(cc, scr);
-              const float2 pr = cmul(cc, e.g[v]);
-              sr += double(pr.x);
-              si += double(pr.y);
-            }
-            dr[k] = sr;
-            di[k] = si;
-            hr[k] = float(sr);
-            him[k] = float(si);
-            lr[k] = float(sr - double(hr[k]));
-            lim[k] = float(si - double(him[k]));
+            const float2 pr = cmul(cc, e.g[v]);
+            sr += double(pr.x);
+            si += double(pr.y);
           }
+          d_r = sr;
+          d_i = si;
+          h_r = float(sr);
+          h_i = float(si);
+          l_r = float(sr - double(h_r));
+          l_i = float(si - double(h_i));
         }
+      };
+      // adds the M taps held in the arrays to LDS pixels li[0..M) (ok[m]:
+      // tap present).  The compare-and-swap form reads each pixel just
+      // before its swap: the wave's LDS operations execute in order, so two
+      // lanes of one wave only race on a pixel when their corners coincide;
+      // failed swaps are retried after the last tap.
+      auto accumulate = [&](auto m_c, const int* li, const bool* ok, const float* h_r,
+                            const float* h_i, const float* l_r, const float* l_i,
+                            const double* d_r, const double* d_i) {
+        constexpr int M = decltype(m_c)::value;
         if constexpr (ACC == acc_f64) {
-  #pragma unroll
-          for (int k = 0; k < halo_chunk; ++k) {
-            if (!ok[k]) continue;
-            __hip_atomic_fetch_add(acc64 + lik[k], dr[k], __ATOMIC_RELAXED,
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            if (!ok[m]) continue;
+            __hip_atomic_fetch_add(acc64 + li[m], d_r[m], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(acc64 + R + lik[k], di[k], __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(acc64 + R + li[m], d_i[m], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         } else if constexpr (ACC == acc_f32_rtn) {
-  #pragma unroll
-          for (int k = 0; k < halo_chunk; ++k) {
-            if (!ok[k]) continue;
-            const int li = lik[k];
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            if (!ok[m]) continue;
             const float orr = __hip_atomic_fetch_add(
-              acc32 + li, hr[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              acc32 + li[m], h_r[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             const float ori = __hip_atomic_fetch_add(
-              acc32 + R + li, him[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              acc32 + R + li[m], h_i[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             float sum, er, ei;
-            two_sum(orr, hr[k], sum, er);
-            two_sum(ori, him[k], sum, ei);
-            __hip_atomic_fetch_add(acc32 + 2 * R + li, er + lr[k], __ATOMIC_RELAXED,
+            two_sum(orr, h_r[m], sum, er);
+            two_sum(ori, h_i[m], sum, ei);
+            __hip_atomic_fetch_add(acc32 + 2 * R + li[m], er + l_r[m], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(acc32 + 3 * R + li, ei + lim[k], __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(acc32 + 3 * R + li[m], ei + l_i[m], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         } else {
-          // pixel li holds (H_re, L_re) at pairs[li] and (H_im, L_im) at
-        // pairs[R + li].  Read the chunk's pixels, then swap each from the
-        // value read, settling each swap before the next (issuing all
-        // swaps first measured slower: the longer window between read and
-        // swap makes more of them fail).  A read may see a pair mid-update;
-        // its swap then fails and the retry starts from the value the swap
-        // returned.
-        constexpr bool has_l = N > 1;
-        unsigned long long er[halo_chunk], ei[halo_chunk];
+          // pixel li holds (H_re, L_re) at pairs[li], (H_im, L_im) at pairs[R + li]
+          constexpr bool has_l = N > 1;
+          unsigned long long er[M], ei[M];
+          bool okr[M], oki[M];
 #pragma unroll
-        for (int k = 0; k < halo_chunk; ++k) {
-          if (!ok[k]) continue;
-          er[k] = pairs[lik[k]];
-          ei[k] = pairs[R + lik[k]];
-        }
+          for (int m = 0; m < M; ++m) {
+            if (!ok[m]) continue;
+            er[m] = pairs[li[m]];
+            ei[m] = pairs[R + li[m]];
+            okr[m] = pair_swap(pairs + li[m], er[m], pair_plus<has_l>(er[m], h_r[m], l_r[m]));
+            oki[m] = pair_swap(pairs + R + li[m], ei[m],
+                               pair_plus<has_l>(ei[m], h_i[m], l_i[m]));
+          }
 #pragma unroll
-        for (int k = 0; k < halo_chunk; ++k) {
-          if (!ok[k]) continue;
-          pair_add<has_l>(pairs + lik[k], er[k], hr[k], lr[k]);
-          pair_add<has_l>(pairs + R + lik[k], ei[k], him[k], lim[k]);
+          for (int m = 0; m < M; ++m) {
+            if (!ok[m]) continue;
+            if (!okr[m]) pair_add<has_l>(pairs + li[m], er[m], h_r[m], l_r[m]);
+            if (!oki[m]) pair_add<has_l>(pairs + R + li[m], ei[m], h_i[m], l_i[m]);
+          }
         }
-      }
-    };
-      int t0 = 0;
-      for (; t0 + halo_chunk <= ntap; t0 += halo_chunk) do_chunk(std::true_type{}, t0);
-      if (t0 < ntap) do_chunk(std::false_type{}, t0);
+      };
+      // any support: taps in slice order (t = Y * sx + X), halo_chunk at a
+      // time; full chunks are straight-line code
+      auto run_chunks = [&]() {
+        const int ntap = sx * sy;
+        int cx = 0, cy = 0;  // (X, Y) of the chunk's first tap
+        auto do_chunk = [&](auto full, int t0) {
+          constexpr bool FULL = decltype(full)::value;
+          bool ok[halo_chunk];
+          int li[halo_chunk];
+          float h_r[halo_chunk], h_i[halo_chunk], l_r[halo_chunk], l_i[halo_chunk];
+          double d_r[halo_chunk], d_i[halo_chunk];
+          float2 c[halo_chunk][N];
+          int Xk[halo_chunk], Yk[halo_chunk];
+#pragma unroll
+          for (int k = 0; k < halo_chunk; ++k) {
+            ok[k] = FULL || t0 + k < ntap;
+            Xk[k] = cx;
+            Yk[k] = cy;
+            li[k] = (ly + cy) * RX + lx + cx;
+            if (++cx == sx) {
+              cx = 0;
+              ++cy;
+            }
+#pragma unroll
+            for (int v = 0; v < N; ++v) {
+              float2 tv = make_float2(0.0f, 0.0f);
+              if (ok[k]) tv = slice[v][t0 + k];
+              c[k][v] = use[v] ? tv : make_float2(0.0f, 0.0f);
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < halo_chunk; ++k)
+            tap_value(c[k], Xk[k], Yk[k], h_r[k], h_i[k], l_r[k], l_i[k], d_r[k], d_i[k]);
+          accumulate(std::integral_constant<int, halo_chunk>{}, li, ok, h_r, h_i, l_r, l_i,
+                     d_r, d_i);
+        };
+        int t0 = 0;
+        for (; t0 + halo_chunk <= ntap; t0 += halo_chunk) do_chunk(std::true_type{}, t0);
+        if (t0 < ntap) do_chunk(std::false_type{}, t0);
+      };
+      run_chunks();
     }
     __syncthreads();  // the next round reuses cls and sched
   }
@@ -2136,56 +2147,59 @@ __global__ __launch_bounds__(halo_block) void k_degrid4(
     }
     double2 a[N];
     for (int v = 0; v < N; ++v) a[v] = make_double2(0.0, 0.0);
-    const int ntap = sx * sy;
-    int cx = 0, cy = 0;
-    auto do_chunk = [&](auto full, int t0) {
-      constexpr bool FULL = decltype(full)::value;
-      bool ok[halo_chunk];
-      int Xk[halo_chunk], Yk[halo_chunk];
-      double2 mv[halo_chunk];
-#pragma unroll
-      for (int k = 0; k < halo_chunk; ++k) {
-        ok[k] = FULL || t0 + k < ntap;
-        Xk[k] = cx;
-        Yk[k] = cy;
-        mv[k] = ok[k] ? mreg[(ly + cy) * RX + lx + cx] : make_double2(0.0, 0.0);
-        if (++cx == sx) {
-          cx = 0;
-          ++cy;
-        }
+    // c' x model for one tap, added to a[]
+    auto add_tap = [&](const float2* cv, double2 m, int X, int Y) {
+      if (grad) {
+        const double phX = phi0x + X * dphix;
+        const double phY = phi0y + Y * dphiy;
+        m = cmul(m, cphase(-phX - phY));
       }
-      float2 c[N][halo_chunk];
 #pragma unroll
-      for (int k = 0; k < halo_chunk; ++k)
-#pragma unroll
-        for (int v = 0; v < N; ++v) {
-          float2 tv = make_float2(0.0f, 0.0f);
-          if (ok[k]) tv = slice[v][t0 + k];
-          c[v][k] = tv;
-        }
-#pragma unroll
-      for (int k = 0; k < halo_chunk; ++k) {
-        if (!ok[k]) continue;
-        double2 m = mv[k];
-        if (grad) {
-          const double phX = phi0x + Xk[k] * dphix;
-          const double phY = phi0y + Yk[k] * dphiy;
-          m = cmul(m, cphase(-phX - phY));
-        }
-#pragma unroll
-        for (int v = 0; v < N; ++v) {
-          if (!use[v]) continue;
-          float2 cc = c[v][k];
-          cc.y *= imf;
-          const double2 p = cmul(cc, m);
-          a[v].x += p.x;
-          a[v].y += p.y;
-        }
+      for (int v = 0; v < N; ++v) {
+        if (!use[v]) continue;
+        float2 cc = cv[v];
+        cc.y *= imf;
+        const double2 p = cmul(cc, m);
+        a[v].x += p.x;
+        a[v].y += p.y;
       }
     };
-    int t0 = 0;
-    for (; t0 + halo_chunk <= ntap; t0 += halo_chunk) do_chunk(std::true_type{}, t0);
-    if (t0 < ntap) do_chunk(std::false_type{}, t0);
+    // any support: taps in slice order, halo_chunk at a time
+    auto run_chunks = [&]() {
+      const int ntap = sx * sy;
+      int cx = 0, cy = 0;
+      auto do_chunk = [&](auto full, int t0) {
+        constexpr bool FULL = decltype(full)::value;
+        bool ok[halo_chunk];
+        int Xk[halo_chunk], Yk[halo_chunk];
+        double2 mv[halo_chunk];
+        float2 c[halo_chunk][N];
+#pragma unroll
+        for (int k = 0; k < halo_chunk; ++k) {
+          ok[k] = FULL || t0 + k < ntap;
+          Xk[k] = cx;
+          Yk[k] = cy;
+          mv[k] = ok[k] ? mreg[(ly + cy) * RX + lx + cx] : make_double2(0.0, 0.0);
+          if (++cx == sx) {
+            cx = 0;
+            ++cy;
+          }
+#pragma unroll
+          for (int v = 0; v < N; ++v) {
+            float2 tv = make_float2(0.0f, 0.0f);
+            if (ok[k]) tv = slice[v][t0 + k];
+            c[k][v] = tv;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < halo_chunk; ++k)
+          if (ok[k]) add_tap(c[k], mv[k], Xk[k], Yk[k]);
+      };
+      int t0 = 0;
+      for (; t0 + halo_chunk <= ntap; t0 += halo_chunk) do_chunk(std::true_type{}, t0);
+      if (t0 < ntap) do_chunk(std::false_type{}, t0);
+    };
+    run_chunks();
     for (int v = 0; v < N; ++v) {
       double2* dst = acc + size_t(i) * N + v;
       if (one_plane) {
@@ -2574,6 +2588,24 @@ size_t halo_lds_bytes(int hx, int hy, int nbins) {
       * (2 * size_t(nbins) + 1 + halo_block / wave + 1 + halo_classes + halo_sched);
 }
 
+// the halo pass after a haloed-tile kernel
+hipError_t launch_halo(
+  hipStream_t s, int mode, size_t nkeys, const Geometry& geo, const CallBuffers& cb,
+  int hx, int hy, void* grid) {
+  const size_t zone = size_t(hy) * tile + size_t(tile - hy) * hx;
+  const size_t count = nkeys * zone;
+  if (count == 0) return hipSuccess;
+  if (mode == 0)
+    hipLaunchKernelGGL(
+      k_halo<double2>, dim3(blocks_for(count)), dim3(block), 0, s, geo, hx, hy,
+      cb.halo, cb.halo_set, nkeys, static_cast<double2*>(grid));
+  else
+    hipLaunchKernelGGL(
+      k_halo<float2>, dim3(blocks_for(count)), dim3(block), 0, s, geo, hx, hy,
+      cb.halo, cb.halo_set, nkeys, static_cast<float2*>(grid));
+  return hipGetLastError();
+}
+
 hipError_t launch_tile_halo(
   hipStream_t s, int npol, int mode, int acc, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
@@ -2610,18 +2642,7 @@ hipError_t launch_tile_halo(
 #undef HPGX_TILE4
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const size_t zone = size_t(hy) * tile + size_t(tile - hy) * hx;
-  const size_t count = nkeys * zone;
-  if (count == 0) return hipSuccess;
-  if (mode == 0)
-    hipLaunchKernelGGL(
-      k_halo<double2>, dim3(blocks_for(count)), dim3(block), 0, s, geo, hx, hy,
-      cb.halo, cb.halo_set, nkeys, static_cast<double2*>(grid));
-  else
-    hipLaunchKernelGGL(
-      k_halo<float2>, dim3(blocks_for(count)), dim3(block), 0, s, geo, hx, hy,
-      cb.halo, cb.halo_set, nkeys, static_cast<float2*>(grid));
-  return hipGetLastError();
+  return launch_halo(s, mode, nkeys, geo, cb, hx, hy, grid);
 }
 
 hipError_t launch_tile_blocal(

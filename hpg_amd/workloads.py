@@ -90,3 +90,78 @@ def random_visibilities(num_vis, npol, grid_size, grid_scale, cf, seed=2,
     if phase:
         rec["phase"] = rng.uniform(-np.pi, np.pi, num_vis)
     return rec
+
+
+# ---- scaled-down forms of the multi-polarization workloads ----------------
+#
+# Full 4x4 Mueller matrices: row r (grid polarization) uses CF Mueller
+# elements 4r..4r+3 for the four visibility polarizations, conjugates the same.
+MUELLER_FULL = [[0, 1, 2, 3], [4, 5, 6, 7], [8, 9, 10, 11], [12, 13, 14, 15]]
+
+
+def wproj_cf(oversampling, radii, w_values, nmueller=16, ncube=1, seed=3,
+             phase_screen=False):
+    """W-projection-style CF groups: group g has support radius radii[g] and
+    taps exp(-rho^2 / (2 (r/2.5)^2)) * exp(i pi w_g rho^2) (rho in pixels
+    from the centre tap), one slightly different amplitude per Mueller
+    element (scaled by 1 / (1 + m)) and cube; with phase_screen, each
+    (cube, Mueller element) is also multiplied by a random unit-modulus
+    screen exp(i phi(x, y))."""
+    rng = np.random.default_rng(seed)
+    groups = []
+    for r, w in zip(radii, w_values):
+        ext = cf_extent(r, oversampling)
+        centre = (r + PADDING) * oversampling
+        t = (np.arange(ext) - centre) / oversampling
+        rho2 = t[None, :] ** 2 + t[:, None] ** 2  # [y][x]
+        sigma = max(r, 1) / 2.5
+        base = np.exp(-rho2 / (2.0 * sigma * sigma)) * np.exp(1j * np.pi * w * rho2)
+        vals = np.empty((ncube, nmueller, ext, ext), dtype=np.complex64)
+        for c in range(ncube):
+            for m in range(nmueller):
+                v = base / (1.0 + m + 0.5 * c)
+                if phase_screen:
+                    v = v * np.exp(1j * rng.uniform(-np.pi, np.pi, v.shape))
+                vals[c, m] = v.astype(np.complex64)
+        groups.append(vals)
+    return CFArray(oversampling, groups)
+
+
+def gaussian_uv_visibilities(num_vis, npol, grid_size, cf, sigma_frac, w_max=1.0,
+                             weight_range=(0.5, 1.5), seed=5):
+    """VisData<npol> records with u, v ~ N(0, (sigma_frac * extent)^2) pixels
+    (grid_scale 1, frequency c), w ~ U(0, w_max) selecting the CF group
+    floor(ngroups * w / w_max), channel (grid cube and CF cube) uniform,
+    values N(0, 1) and weights uniform in weight_range."""
+    rng = np.random.default_rng(seed)
+    rec = np.zeros(num_vis, dtype=visdata_dtype(npol))
+    v = rng.standard_normal((num_vis, npol, 2)).astype(np.float32)
+    rec["visibilities"] = v[..., 0] + 1j * v[..., 1]
+    lo, hi = weight_range
+    rec["weights"] = rng.uniform(lo, hi, (num_vis, npol)).astype(np.float32)
+    rec["frequency"] = C_LIGHT
+    for d in range(2):
+        rec["uvw"][:, d] = rng.normal(0.0, sigma_frac * grid_size[d], num_vis)
+    w = rng.uniform(0.0, w_max, num_vis)
+    rec["uvw"][:, 2] = w
+    ng = cf.num_groups()
+    grp = np.minimum((ng * w / w_max).astype(np.int64), ng - 1)
+    rec["cf_index"][:, 1] = grp
+    ncube = min(grid_size[3], min(cf.extents(g)[3] for g in range(ng)))
+    cube = rng.integers(0, ncube, num_vis)
+    rec["grid_cube"] = cube
+    rec["cf_index"][:, 0] = cube
+    return rec
+
+
+def point_source_image(size, npol, num_sources=200, noise=1e-4, seed=11):
+    """(npol, size, size) float32 sky: point sources at random pixels with
+    log-uniform flux in [1e-3, 1], plus N(0, noise^2) noise."""
+    rng = np.random.default_rng(seed)
+    img = (noise * rng.standard_normal((npol, size, size))).astype(np.float32)
+    ys = rng.integers(0, size, num_sources)
+    xs = rng.integers(0, size, num_sources)
+    flux = np.exp(rng.uniform(np.log(1e-3), 0.0, num_sources)).astype(np.float32)
+    for p in range(npol):
+        img[p, ys, xs] += flux
+    return img
