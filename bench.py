@@ -381,7 +381,7 @@ def main():
                    "device_ms_per_call": degrid_ms,
                    "bytes_per_call": degrid_bytes(NUM_VIS),
                    "hbm_frac": degrid_bytes(NUM_VIS) / (degrid_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                   "kernel": "record staging copy + k_bin + k_degrid4 + k_degrid_finish"},
+                   "kernel": "record staging copy + k_bin + k_degrid4"},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -534,27 +534,43 @@ int run_degrid_binned(hpgx_state_t st, size_t n, bool do_grid, const void* recor
     HPGX_HIP(st, dev_alloc(cb.degrid_acc, acc_need * sizeof(double2)));
     cb.degrid_acc_capacity = acc_need;
   }
-  HPGX_HIP(st, hipMemsetAsync(cb.degrid_acc, 0, n * np * sizeof(double2), st->stream));
+  // one grid row whose Mueller rows (both w signs) index some CF element:
+  // every record inside the grid has exactly one entry, and the tile kernel
+  // finishes its prediction
+  bool one_plane = st->gsize[2] == 1;
+  if (one_plane) {
+    bool m_any = false, mc_any = false;
+    for (unsigned c = 0; c < st->npol; ++c) {
+      m_any |= st->mu.M[0][c] >= 0;
+      mc_any |= st->mu.Mc[0][c] >= 0;
+    }
+    one_plane = m_any && mc_any;
+  }
+  if (!one_plane)
+    HPGX_HIP(st, hipMemsetAsync(cb.degrid_acc, 0, n * np * sizeof(double2), st->stream));
   const size_t epr = entries_per_record(st);
   const size_t block_cap = hpgx::bin_block_records() * epr;
   const size_t chunk = std::max<size_t>(1, st->pass_blocks) * hpgx::bin_block_records();
   const size_t rec_bytes = visdata_size(st->npol);
   const void* rec_base = records ? records : cb.vis;
-  const bool one_plane = st->gsize[2] == 1;
   for (size_t off = 0; off < n; off += chunk) {
     const size_t m = std::min(chunk, n - off);
     CallBuffers c = cb;
     c.vis = const_cast<char*>(static_cast<const char*>(rec_base)) + off * rec_bytes;
+    c.gvis = cb.gvis + off * np;
     HPGX_HIP(st, hpgx::launch_bin(
                    st->stream, np, m, geo, st->cf, st->mu, false, false,
                    hpgx::GvisSource::Values, nullptr, c, block_cap, st->weights, true, true));
     HPGX_HIP(st, hpgx::launch_degrid_tiles(
                    st->stream, np, m, st->nkeys, geo, st->cf, st->mu, c, block_cap,
-                   halo_x(st), halo_y(st), st->model, cb.degrid_acc + off * np, one_plane));
+                   halo_x(st), halo_y(st), st->model, cb.degrid_acc + off * np, one_plane,
+                   do_grid, static_cast<char*>(cb.vis) + off * rec_bytes,
+                   pred64 ? pred64 + off * np : nullptr));
   }
-  HPGX_HIP(st, hpgx::launch_degrid_finish(
-                 st->stream, np, n, geo, st->cf, st->mu, cb.degrid_acc, do_grid, records, cb,
-                 pred64));
+  if (!one_plane)
+    HPGX_HIP(st, hpgx::launch_degrid_finish(
+                   st->stream, np, n, geo, st->cf, st->mu, cb.degrid_acc, do_grid, records,
+                   cb, pred64));
   return HPGX_SUCCESS;
 }
 

@@ -163,13 +163,17 @@ hipError_t launch_tile_blocal(
   hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
   void* grid);
-// binned degridding: per-record sums over the footprint from the model's
-// LDS tiles (entries of launch_bin with degrid_pass), stored into acc when
-// each record has one plane, else added (acc zeroed beforehand)
+// binned degridding from the model's LDS tiles (entries of launch_bin with
+// degrid_pass).  one_plane (every record has exactly one entry): the
+// predictions are finished in the same kernel -- FP64 values to pred64, or
+// values written into out_recs (residuals, and gvis in cb, when do_grid).
+// Otherwise per-record sums are added into acc (zeroed beforehand) for
+// launch_degrid_finish.
 hipError_t launch_degrid_tiles(
   hipStream_t s, int npol, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
-  int hx, int hy, const double2* model, double2* acc, bool one_plane);
+  int hx, int hy, const double2* model, double2* acc, bool one_plane, bool do_grid,
+  void* out_recs, double2* pred64);
 // predictions from those sums: into the records (cb.vis; residuals and gvis
 // when do_grid) or, with pred64, as FP64 values; records read from
 // `records` when given, else cb.vis

@@ -1650,7 +1650,9 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
       // time; full chunks are straight-line code
       auto run_chunks = [&]() {
         const int ntap = sx * sy;
-        int cx = 0, cy = 0;  // (X, Y) of the chunk's first tap
+        int cx = 0, cy = 0;          // (X, Y) of the chunk's first tap
+        int lc = ly * RX + lx;       // its LDS pixel
+        const int wrap = RX - sx;    // pixel step from a row end to the next row
         auto do_chunk = [&](auto full, int t0) {
           constexpr bool FULL = decltype(full)::value;
           bool ok[halo_chunk];
@@ -1664,11 +1666,11 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
             ok[k] = FULL || t0 + k < ntap;
             Xk[k] = cx;
             Yk[k] = cy;
-            li[k] = (ly + cy) * RX + lx + cx;
-            if (++cx == sx) {
-              cx = 0;
-              ++cy;
-            }
+            li[k] = lc;
+            const bool eol = ++cx == sx;
+            cx = eol ? 0 : cx;
+            cy += eol ? 1 : 0;
+            lc += eol ? 1 + wrap : 1;
 #pragma unroll
             for (int v = 0; v < N; ++v) {
               float2 tv = make_float2(0.0f, 0.0f);
@@ -2055,9 +2057,17 @@ __global__ __launch_bounds__(halo_block) void k_degrid4(
   Geometry geo, const GroupInfo* __restrict__ groups,
   const float2* __restrict__ taps, Mueller mu, const unsigned* __restrict__ off_table,
   int nbins, size_t block_cap, const Entry4<N>* __restrict__ entries,
-  const unsigned* __restrict__ entry_vis, const VisRec<N>* __restrict__ recs,
+  const unsigned* __restrict__ entry_vis, const VisRec<N>* recs,
   int hx, int hy, const double2* __restrict__ model, double2* __restrict__ acc,
-  int one_plane) {
+  int one_plane, const double2* __restrict__ wsum, int do_grid, VisRec<N>* out_recs,
+  float2* __restrict__ gvis, double2* __restrict__ pred64) {
+  // one_plane: each record has exactly one entry, so the lane finishes the
+  // prediction itself (the end of degrid_vis and the degridding branch of
+  // VisibilityGridder::kernel, src/hpg_core.hpp:643-682, 1136-1169): FP64
+  // predictions to pred64, or values written into out_recs (residuals and
+  // gvis when do_grid); recs and out_recs may be the same records, each
+  // lane reading its record's fields before writing them.  Otherwise the
+  // per-plane sums are added into acc for k_degrid_finish.
 
   extern __shared__ double smem[];
   const int RX = tile + hx;
@@ -2200,13 +2210,38 @@ __global__ __launch_bounds__(halo_block) void k_degrid4(
       if (t0 < ntap) do_chunk(std::false_type{}, t0);
     };
     run_chunks();
-    for (int v = 0; v < N; ++v) {
-      double2* dst = acc + size_t(i) * N + v;
-      if (one_plane) {
-        *dst = a[v];
-      } else {
+    if (!one_plane) {
+      for (int v = 0; v < N; ++v) {
+        double2* dst = acc + size_t(i) * N + v;
         atomicAdd(&dst->x, a[v].x);
         atomicAdd(&dst->y, a[v].y);
+      }
+      continue;
+    }
+    double2 W[N];
+    degrid_norms<N>(geo, g, wsum, mu, pos_w, unsigned(cfcube), fxi, fyi, W);
+    const double2 phasor = cphase(recs[i].phase);
+    const double2 conj_phasor = make_double2(phasor.x, -phasor.y);
+    double2 pred_d[N];
+    for (int v = 0; v < N; ++v) {
+      const double2 d = (W[v].x != 0.0 || W[v].y != 0.0) ? W[v] : make_double2(1.0, 0.0);
+      pred_d[v] = cmul(kdiv(a[v], d), conj_phasor);
+    }
+    if (pred64) {
+      for (int v = 0; v < N; ++v) pred64[size_t(i) * N + v] = pred_d[v];
+      continue;
+    }
+    for (int v = 0; v < N; ++v) {
+      const float2 pred = narrow(pred_d[v]);
+      if (do_grid) {
+        float2 val = recs[i].vis[v];
+        val.x -= pred.x;
+        val.y -= pred.y;
+        const float wv = recs[i].w[v];
+        out_recs[i].vis[v] = val;
+        gvis[size_t(i) * N + v] = prephase(val, phasor, wv);
+      } else {
+        out_recs[i].vis[v] = pred;
       }
     }
   }
@@ -2683,7 +2718,8 @@ size_t degrid_lane_max_taps() { return size_t(degrid_lane_taps); }
 hipError_t launch_degrid_tiles(
   hipStream_t s, int npol, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
-  int hx, int hy, const double2* model, double2* acc, bool one_plane) {
+  int hx, int hy, const double2* model, double2* acc, bool one_plane, bool do_grid,
+  void* out_recs, double2* pred64) {
   if (nkeys == 0 || n == 0) return hipSuccess;
   if (hx < 0 || hy < 0 || hx >= tile || hy >= tile) return hipErrorInvalidValue;
   const int nb = bin_blocks_for(n);
@@ -2694,7 +2730,8 @@ hipError_t launch_degrid_tiles(
     k_degrid4<NP>, dim3(unsigned(nkeys)), dim3(halo_block), lds, s, geo,       \
     cf.groups, cf.taps, mu, cb.off_table, nb, block_cap,                       \
     static_cast<const Entry4<NP>*>(cb.stream_entries), cb.entry_vis,           \
-    static_cast<const VisRec<NP>*>(cb.vis), hx, hy, model, acc, int(one_plane))
+    static_cast<const VisRec<NP>*>(cb.vis), hx, hy, model, acc, int(one_plane),  \
+    cf.wsum, int(do_grid), static_cast<VisRec<NP>*>(out_recs), cb.gvis, pred64)
   switch (npol) {
   case 1: HPGX_DEGRID4(1); break;
   case 2: HPGX_DEGRID4(2); break;
