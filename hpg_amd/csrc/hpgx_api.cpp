@@ -520,10 +520,12 @@ int upload_model(hpgx_state_t st, const void* values, int layout, int memory) {
 // gridding / degridding pipeline on records already in cb.vis; the records
 // are gridded in chunks whose bin entries fit the entry buffer
 // binned degridding of n records (cb.vis, or `records` read in place):
-// predicted values into the records (residuals and gvis when do_grid), or
-// FP64 predictions into pred64.  Requires use_halo(st) and a model.
+// FP64 predictions into pred64, or predicted values (residuals when
+// do_grid) written into `out_records` (cb.vis when null and records is
+// null; not written when only records is given), gvis when do_grid.
+// Requires use_halo(st) and a model.
 int run_degrid_binned(hpgx_state_t st, size_t n, bool do_grid, const void* records,
-                      double2* pred64) {
+                      double2* pred64, void* out_records = nullptr) {
   const Geometry geo = st->geometry();
   CallBuffers& cb = st->cb;
   const int np = int(st->npol);
@@ -553,6 +555,7 @@ int run_degrid_binned(hpgx_state_t st, size_t n, bool do_grid, const void* recor
   const size_t chunk = std::max<size_t>(1, st->pass_blocks) * hpgx::bin_block_records();
   const size_t rec_bytes = visdata_size(st->npol);
   const void* rec_base = records ? records : cb.vis;
+  void* out_base = out_records ? out_records : (records ? nullptr : cb.vis);
   for (size_t off = 0; off < n; off += chunk) {
     const size_t m = std::min(chunk, n - off);
     CallBuffers c = cb;
@@ -564,20 +567,22 @@ int run_degrid_binned(hpgx_state_t st, size_t n, bool do_grid, const void* recor
     HPGX_HIP(st, hpgx::launch_degrid_tiles(
                    st->stream, np, m, st->nkeys, geo, st->cf, st->mu, c, block_cap,
                    halo_x(st), halo_y(st), st->model, cb.degrid_acc + off * np, one_plane,
-                   do_grid, static_cast<char*>(cb.vis) + off * rec_bytes,
+                   do_grid, out_base ? static_cast<char*>(out_base) + off * rec_bytes : nullptr,
                    pred64 ? pred64 + off * np : nullptr));
   }
   if (!one_plane)
     HPGX_HIP(st, hpgx::launch_degrid_finish(
-                   st->stream, np, n, geo, st->cf, st->mu, cb.degrid_acc, do_grid, records,
-                   cb, pred64));
+                   st->stream, np, n, geo, st->cf, st->mu, cb.degrid_acc, do_grid, rec_base,
+                   out_base, cb, pred64));
   return HPGX_SUCCESS;
 }
 
-// records: device records read in place (nullptr: the staged copy cb.vis)
+// records: device records read in place (nullptr: the staged copy cb.vis);
+// out_records: where the binned degridder writes predicted / residual
+// values when records are read in place (nullptr: not written)
 int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
              bool do_grid, hpgx::GvisSource src, const double2* residuals,
-             const void* records = nullptr) {
+             const void* records = nullptr, void* out_records = nullptr) {
   const Geometry geo = st->geometry();
   CallBuffers& cb = st->cb;
   void* rec_base = records ? const_cast<void*>(records) : cb.vis;
@@ -585,7 +590,7 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
   bool gvis_ready = false;
   if (do_degrid) {
     if (st->model && use_halo(st)) {
-      if (int rc = run_degrid_binned(st, n, do_grid, nullptr, nullptr)) return rc;
+      if (int rc = run_degrid_binned(st, n, do_grid, records, nullptr, out_records)) return rc;
     } else {
       HPGX_HIP(st, hpgx::launch_degrid(
                      st->stream, np, n, geo, st->cf, st->mu, st->model, do_grid, cb));
@@ -1061,14 +1066,34 @@ int hpgx_grid_visibilities(
     return HPGX_SUCCESS;
   }
   if (int rc = ensure_call_buffers(st)) return rc;
-  // device records are read in place unless degridding rewrites them
-  const bool in_place = memory == HPGX_DEVICE && !do_degrid;
-  if (!in_place)
+  // Device records are read in place unless degridding rewrites them.  The
+  // binned degridder writes the rewritten records straight into
+  // visibilities_out (after a copy of the input when they differ), or
+  // nowhere when they are not returned; other degridders use the staged
+  // copy.
+  const bool dev = memory == HPGX_DEVICE;
+  const bool binned_degrid = do_degrid && st->model && use_halo(st);
+  const void* records = nullptr;  // nullptr: the staged copy
+  void* out_records = nullptr;
+  if (dev && !do_degrid) {
+    records = vis;
+  } else if (dev && binned_degrid) {
+    if (return_visibilities) {
+      if (vis_out != vis)
+        HPGX_HIP(st, hipMemcpyAsync(vis_out, vis, n * visdata_size(npol),
+                                    hipMemcpyDeviceToDevice, st->stream));
+      records = vis_out;
+      out_records = vis_out;
+    } else {
+      records = vis;
+    }
+  } else {
     if (int rc = stage_records(st, vis, n, memory)) return rc;
+  }
   if (int rc = run_grid(st, n, update_grid_weights, do_degrid, do_grid,
-                        hpgx::GvisSource::Values, nullptr, in_place ? vis : nullptr))
+                        hpgx::GvisSource::Values, nullptr, records, out_records))
     return rc;
-  const void* rec_base = in_place ? vis : st->cb.vis;
+  const void* rec_base = records ? records : st->cb.vis;
   if (return_visibilities && vis_out != rec_base) {
     HPGX_HIP(st, hipMemcpyAsync(
                    vis_out, rec_base, n * visdata_size(npol),

@@ -174,13 +174,13 @@ hipError_t launch_degrid_tiles(
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
   int hx, int hy, const double2* model, double2* acc, bool one_plane, bool do_grid,
   void* out_recs, double2* pred64);
-// predictions from those sums: into the records (cb.vis; residuals and gvis
-// when do_grid) or, with pred64, as FP64 values; records read from
-// `records` when given, else cb.vis
+// predictions from those sums for the records at `records`: FP64 values
+// to pred64, or values written into out_records when not null (residuals,
+// and gvis in cb, when do_grid)
 hipError_t launch_degrid_finish(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, const double2* acc, bool do_grid, const void* records,
-  CallBuffers& cb, double2* pred64);
+  void* out_records, CallBuffers& cb, double2* pred64);
 // degridding; with pred64 the FP64 predictions are written there and the
 // records are only read, from `records` when given (else cb.vis; `records`
 // requires supports of at most degrid_lane_max_taps() taps)

@@ -2064,10 +2064,10 @@ __global__ __launch_bounds__(halo_block) void k_degrid4(
   // one_plane: each record has exactly one entry, so the lane finishes the
   // prediction itself (the end of degrid_vis and the degridding branch of
   // VisibilityGridder::kernel, src/hpg_core.hpp:643-682, 1136-1169): FP64
-  // predictions to pred64, or values written into out_recs (residuals and
-  // gvis when do_grid); recs and out_recs may be the same records, each
-  // lane reading its record's fields before writing them.  Otherwise the
-  // per-plane sums are added into acc for k_degrid_finish.
+  // predictions to pred64, or values written into out_recs when given
+  // (residuals, and gvis, when do_grid); recs and out_recs may be the same
+  // records, each lane reading its record's fields before writing them.
+  // Otherwise the per-plane sums are added into acc for k_degrid_finish.
 
   extern __shared__ double smem[];
   const int RX = tile + hx;
@@ -2238,9 +2238,9 @@ __global__ __launch_bounds__(halo_block) void k_degrid4(
         val.x -= pred.x;
         val.y -= pred.y;
         const float wv = recs[i].w[v];
-        out_recs[i].vis[v] = val;
+        if (out_recs) out_recs[i].vis[v] = val;
         gvis[size_t(i) * N + v] = prephase(val, phasor, wv);
-      } else {
+      } else if (out_recs) {
         out_recs[i].vis[v] = pred;
       }
     }
@@ -2257,7 +2257,7 @@ __global__ __launch_bounds__(block) void k_degrid_finish(
   int do_grid, const VisRec<N>* recs_in, VisRec<N>* recs, float2* __restrict__ gvis,
   double2* __restrict__ pred64) {
   // recs_in and recs may be the same records: each thread reads its
-  // record's fields before writing them back
+  // record's fields before writing them back; recs null: values not written
   const size_t i = blockIdx.x * (size_t)block + threadIdx.x;
   if (i >= n) return;
   const VisRec<N>& r = recs_in[i];
@@ -2285,16 +2285,15 @@ __global__ __launch_bounds__(block) void k_degrid_finish(
     for (int v = 0; v < N; ++v) pred64[i * N + v] = pred_d[v];
     return;
   }
-  VisRec<N>& o = recs[i];
   for (int v = 0; v < N; ++v) {
     if (do_grid) {
       float2 val = r.vis[v];
       val.x -= pred[v].x;
       val.y -= pred[v].y;
-      o.vis[v] = val;
+      if (recs) recs[i].vis[v] = val;
       gvis[i * N + v] = prephase(val, phasor, r.w[v]);
-    } else {
-      o.vis[v] = pred[v];
+    } else if (recs) {
+      recs[i].vis[v] = pred[v];
     }
   }
 }
@@ -2746,14 +2745,14 @@ hipError_t launch_degrid_tiles(
 hipError_t launch_degrid_finish(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, const double2* acc, bool do_grid, const void* records,
-  CallBuffers& cb, double2* pred64) {
+  void* out_records, CallBuffers& cb, double2* pred64) {
   if (n == 0) return hipSuccess;
-  const void* in = records ? records : cb.vis;
+  const void* in = records;
 #define HPGX_DFIN(NP)                                                          \
   hipLaunchKernelGGL(                                                          \
     k_degrid_finish<NP>, dim3(blocks_for(n)), dim3(block), 0, s, n, geo,       \
     cf.groups, cf.wsum, mu, acc, int(do_grid),                                 \
-    static_cast<const VisRec<NP>*>(in), static_cast<VisRec<NP>*>(cb.vis),      \
+    static_cast<const VisRec<NP>*>(in), static_cast<VisRec<NP>*>(out_records), \
     cb.gvis, pred64)
   switch (npol) {
   case 1: HPGX_DFIN(1); break;

@@ -182,9 +182,10 @@ HPGX_API int hpgx_has_model(hpgx_state_t state);
  * for records inside the grid -- are written to visibilities_out (same
  * memory kind; may equal visibilities) once the stream reaches that point;
  * they are valid after the next hpgx_fence.  HPGX_HOST records are copied
- * before the call returns; HPGX_DEVICE records are read in place by the
- * queued kernels unless do_degrid is set, so they must be ready when the
- * call is made and stay unchanged until the next hpgx_fence. */
+ * before the call returns.  HPGX_DEVICE records may be read in place by the
+ * queued kernels, so they must be ready when the call is made and stay
+ * unchanged until the next hpgx_fence; they are never written (rewritten
+ * values go to visibilities_out or to the state's own copy). */
 HPGX_API int hpgx_grid_visibilities(
   hpgx_state_t state, const void* visibilities, size_t num_visibilities,
   unsigned num_polarizations, int update_grid_weights, int do_degrid,

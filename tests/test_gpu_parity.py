@@ -425,3 +425,51 @@ def test_device_resident_records(hpg, oracle_mod):
     gs2.grid_visibilities(hpg.Device.Serial, dev)
     a, b = gs1.grid_values(), gs2.grid_values()
     np.testing.assert_allclose(b, a, rtol=0, atol=1e-12 * np.abs(a).max())
+
+
+@pytest.mark.parametrize("case", [0, 2, 3])
+def test_device_resident_degridding(hpg, oracle_mod, case):
+    """Degridding with records in device memory (read in place; the input
+    tensor is never written) gives the values and grids of host records:
+    predicted values, residuals with gridding, and gridding of residuals
+    without returning them.  Case 0: one grid row (predictions finished in
+    the tile kernel); case 2: two grid rows (per-record sums); case 3: a
+    35-pixel support (staged copy, one wave per record)."""
+    import torch
+    from hpg_amd import workloads
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[case]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=80 + case)
+    rec = workloads.random_visibilities(600, npol, gsz, (1.0, 1.0), cf, seed=81 + case,
+                                        phase=True, margin=-2)
+    rng = np.random.default_rng(82 + case)
+    shape = (gsz[3], gsz[2], gsz[1], gsz[0])
+    model = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    dev = torch.from_numpy(rec.view(np.uint8).copy()).cuda()
+    before = dev.clone()
+
+    def pair():
+        out = []
+        for _ in range(2):
+            gs, _ = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), M, cf, 0, Mc)
+            gs.set_model(hpg.Device.Serial, model)
+            out.append(gs)
+        return out
+
+    g_host, g_dev = pair()
+    p_host = g_host.degrid_get_predicted_visibilities(hpg.Device.Serial, rec)
+    p_dev = g_dev.degrid_get_predicted_visibilities(hpg.Device.Serial, dev)
+    assert p_dev.cpu().numpy().view(rec.dtype).tobytes() == p_host.tobytes()
+    g_host, g_dev = pair()
+    r_host = g_host.degrid_grid_get_residual_visibilities(hpg.Device.Serial, rec)
+    r_dev = g_dev.degrid_grid_get_residual_visibilities(hpg.Device.Serial, dev)
+    assert r_dev.cpu().numpy().view(rec.dtype).tobytes() == r_host.tobytes()
+    a, b = g_host.grid_values(), g_dev.grid_values()
+    np.testing.assert_allclose(b, a, rtol=0, atol=1e-12 * np.abs(a).max())
+    g_host, g_dev = pair()
+    g_host.degrid_grid_visibilities(hpg.Device.Serial, rec)
+    g_dev.degrid_grid_visibilities(hpg.Device.Serial, dev)
+    a, b = g_host.grid_values(), g_dev.grid_values()
+    np.testing.assert_allclose(b, a, rtol=0, atol=1e-12 * np.abs(a).max())
+    np.testing.assert_allclose(g_dev.grid_weights(), g_host.grid_weights(), rtol=1e-14)
+    g_dev.fence()
+    assert torch.equal(dev, before)
