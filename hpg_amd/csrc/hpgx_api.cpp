@@ -212,11 +212,20 @@ int halo_x(hpgx_state_t st) { return std::max(1, st->cf.max_sx) - 1; }
 int halo_y(hpgx_state_t st) { return std::max(1, st->cf.max_sy) - 1; }
 
 // haloed tiles: supports narrower than a tile, tile + halo within LDS
-bool use_halo(hpgx_state_t st) {
-  if (st->gridder != 4 || !use_block_local(st)) return false;
+bool halo_ok(hpgx_state_t st) {
+  if (st->gridder != 4) return false;
   const int hx = halo_x(st), hy = halo_y(st);
   return hx < hpgx::halo_tile_limit() && hy < hpgx::halo_tile_limit()
     && hpgx::halo_lds_bytes(hx, hy, 1) <= 65536;
+}
+
+// haloed tiles fed by block-local bins (grids of up to bin_keys_limit keys)
+bool use_halo(hpgx_state_t st) { return halo_ok(st) && use_block_local(st); }
+
+// haloed tiles fed by global bins (larger grids: counters are then
+// rarely contended)
+bool use_halo_global(hpgx_state_t st) {
+  return halo_ok(st) && st->nkeys > size_t(hpgx::bin_keys_limit());
 }
 
 // bin blocks per haloed-tile pass: the per-block segment tables share the
@@ -229,7 +238,7 @@ size_t halo_bins_limit(hpgx_state_t st) {
 
 size_t entries_per_record(hpgx_state_t st) {
   const size_t planes = size_t(std::max(1u, st->gsize[2]));
-  if (use_halo(st)) return planes;
+  if (use_halo(st) || use_halo_global(st)) return planes;
   return planes
     * hpgx::tiles_per_footprint(std::max(1, st->cf.max_sx), std::max(1, st->cf.max_sy));
 }
@@ -266,7 +275,7 @@ int ensure_call_buffers(hpgx_state_t st) {
     // bin entries for the whole batch, up to the entry memory budget; larger
     // batches are gridded in chunks (run_grid)
     const size_t epr = entries_per_record(st);
-    const bool halo = use_halo(st);
+    const bool halo = use_halo(st) || use_halo_global(st);
     const size_t per_entry = st->gridder == 1
       ? sizeof(unsigned)
       : (halo ? hpgx::halo_entry_bytes(int(st->npol)) : hpgx::entry_bytes(int(st->npol)))
@@ -283,28 +292,27 @@ int ensure_call_buffers(hpgx_state_t st) {
         std::max<size_t>(1, std::min({blocks_needed, budget / block_cap, bins_max}));
       need = blocks * block_cap;
       st->pass_blocks = blocks;
-      if (halo) {
-        const size_t hp =
-          st->nkeys * hpgx::halo_pixels_per_key(halo_x(st), halo_y(st));
-        if (hp > cb.halo_capacity) {
-          dev_free(cb.halo);
-          cb.halo_capacity = 0;
-          HPGX_HIP(st, dev_alloc(cb.halo, hp * sizeof(double2)));
-          cb.halo_capacity = hp;
-        }
-        if (st->nkeys > cb.halo_set_capacity) {
-          dev_free(cb.halo_set);
-          cb.halo_set_capacity = 0;
-          HPGX_HIP(st, dev_alloc(cb.halo_set, st->nkeys * sizeof(unsigned)));
-          cb.halo_set_capacity = st->nkeys;
-        }
-      }
       const size_t tab = blocks * (st->nkeys + 1);
       if (tab > cb.off_table_capacity) {
         dev_free(cb.off_table);
         cb.off_table_capacity = 0;
         HPGX_HIP(st, dev_alloc(cb.off_table, tab * sizeof(unsigned)));
         cb.off_table_capacity = tab;
+      }
+    }
+    if (halo) {
+      const size_t hp = st->nkeys * hpgx::halo_pixels_per_key(halo_x(st), halo_y(st));
+      if (hp > cb.halo_capacity) {
+        dev_free(cb.halo);
+        cb.halo_capacity = 0;
+        HPGX_HIP(st, dev_alloc(cb.halo, hp * sizeof(double2)));
+        cb.halo_capacity = hp;
+      }
+      if (st->nkeys > cb.halo_set_capacity) {
+        dev_free(cb.halo_set);
+        cb.halo_set_capacity = 0;
+        HPGX_HIP(st, dev_alloc(cb.halo_set, st->nkeys * sizeof(unsigned)));
+        cb.halo_set_capacity = st->nkeys;
       }
     }
     if (need > cb.entries_capacity) {
@@ -638,14 +646,20 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
     const double2* res = residuals ? residuals + off * np : nullptr;
     HPGX_HIP(st, hipMemsetAsync(c.hist, 0, (st->nkeys + 1) * sizeof(unsigned), st->stream));
     if (st->gridder != 1) {
+      const bool halo = use_halo_global(st);
       HPGX_HIP(st, hpgx::launch_count(
                      st->stream, np, m, geo, st->cf, st->mu, update_weights, true, c,
-                     st->weights));
+                     st->weights, halo));
       HPGX_HIP(st, hpgx::launch_scan(st->stream, st->nkeys, c));
       HPGX_HIP(st, hpgx::launch_emit(
-                     st->stream, np, m, geo, st->cf, st->mu, gvis_ready, src, res, c));
-      HPGX_HIP(st, hpgx::launch_tile_stream(
-                     st->stream, np, mode, st->nkeys, geo, st->cf, st->mu, c, st->grid));
+                     st->stream, np, m, geo, st->cf, st->mu, gvis_ready, src, res, c, halo));
+      if (halo)
+        HPGX_HIP(st, hpgx::launch_tile_halo(
+                       st->stream, np, mode, st->fp32_tile, m, st->nkeys, geo, st->cf, st->mu,
+                       c, 0, halo_x(st), halo_y(st), st->grid, true));
+      else
+        HPGX_HIP(st, hpgx::launch_tile_stream(
+                       st->stream, np, mode, st->nkeys, geo, st->cf, st->mu, c, st->grid));
     } else {
       HPGX_HIP(st, hpgx::launch_prep(
                      st->stream, np, m, geo, st->cf, st->mu, update_weights, true,

@@ -126,15 +126,18 @@ hipError_t launch_scatter(
 hipError_t launch_tile_grid(
   hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid);
+// global bins; halo_entries: one Entry4 per record and plane in the bin of
+// the tile holding the support corner (for launch_tile_halo with
+// global_bins), else one Entry per touched tile
 hipError_t launch_count(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool update_weights, bool do_grid, CallBuffers& cb,
-  double* weights);
+  double* weights, bool halo_entries = false);
 size_t entry_bytes(int npol);
 hipError_t launch_emit(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool gvis_ready, GvisSource src, const double2* residuals,
-  CallBuffers& cb);
+  CallBuffers& cb, bool halo_entries = false);
 hipError_t launch_tile_stream(
   hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid);
@@ -158,7 +161,7 @@ size_t halo_lds_bytes(int hx, int hy, int nbins);
 hipError_t launch_tile_halo(
   hipStream_t s, int npol, int mode, int acc, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
-  int hx, int hy, void* grid);
+  int hx, int hy, void* grid, bool global_bins = false);
 hipError_t launch_tile_blocal(
   hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,

@@ -547,7 +547,9 @@ struct Entry {
   float2 g[N];
 };
 
-template <int N>
+// HALO: one entry per plane, in the bin of the tile holding the support
+// corner (the haloed-tile pipeline); else one per touched tile
+template <int N, bool HALO>
 __global__ __launch_bounds__(block) void k_count(
   const VisRec<N>* __restrict__ recs, size_t n, Geometry geo,
   const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
@@ -593,8 +595,8 @@ __global__ __launch_bounds__(block) void k_count(
       }
       if (c.inside && do_grid) {
         const int ntiles = geo.ntx * geo.nty;
-        const int tx0 = tiles_begin(c.gx), tx1 = tiles_end(c.gx, g.sx);
-        const int ty0 = tiles_begin(c.gy), ty1 = tiles_end(c.gy, g.sy);
+        const int tx0 = tiles_begin(c.gx), tx1 = HALO ? tx0 + 1 : tiles_end(c.gx, g.sx);
+        const int ty0 = tiles_begin(c.gy), ty1 = HALO ? ty0 + 1 : tiles_end(c.gy, g.sy);
         for (int gpol = 0; gpol < geo.nr; ++gpol) {
           if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
           const size_t base = size_t(gcube * geo.nr + gpol) * ntiles;
@@ -621,13 +623,13 @@ __global__ __launch_bounds__(block) void k_count(
   }
 }
 
-template <int N>
+template <int N, bool HALO>
 __global__ __launch_bounds__(block) void k_emit(
   const VisRec<N>* __restrict__ recs, size_t n, Geometry geo,
   const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
   Mueller mu, int gvis_ready, int adjoint, const float2* __restrict__ gvis_in,
   const double2* __restrict__ residuals, unsigned* __restrict__ cursor,
-  Entry<N>* __restrict__ entries, unsigned* __restrict__ entry_vis) {
+  void* __restrict__ entries_out, unsigned* __restrict__ entry_vis) {
 
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -668,15 +670,29 @@ __global__ __launch_bounds__(block) void k_emit(
     }
   }
   const int ntiles = geo.ntx * geo.nty;
-  const int tx0 = tiles_begin(c.gx), tx1 = tiles_end(c.gx, g.sx);
-  const int ty0 = tiles_begin(c.gy), ty1 = tiles_end(c.gy, g.sy);
+  const int tx0 = tiles_begin(c.gx), tx1 = HALO ? tx0 + 1 : tiles_end(c.gx, g.sx);
+  const int ty0 = tiles_begin(c.gy), ty1 = HALO ? ty0 + 1 : tiles_end(c.gy, g.sy);
   for (int gpol = 0; gpol < geo.nr; ++gpol) {
     if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
     const size_t base = size_t(gcube * geo.nr + gpol) * ntiles;
     for (int ty = ty0; ty < ty1; ++ty)
       for (int tx = tx0; tx < tx1; ++tx) {
         const unsigned pos = atomicAdd(cursor + base + ty * geo.ntx + tx, 1u);
-        entries[pos] = e;
+        if constexpr (HALO) {
+          // the haloed-tile entry (layout of Entry4 below): corner relative
+          // to the tile, fine offsets, flags
+          const unsigned hpos = unsigned(c.gx - tx0 * tile)
+            | (unsigned(c.gy - ty0 * tile) << 6) | ((e.h.fpk & 0xffffu) << 12)
+            | (pos_w ? (1u << 28) : 0u) | (grad ? (1u << 29) : 0u);
+          unsigned* q = static_cast<unsigned*>(entries_out)
+            + size_t(pos) * (2 + 2 * N);
+          q[0] = hpos;
+          q[1] = e.h.gpk;
+          float2* qg = reinterpret_cast<float2*>(q + 2);
+          for (int v = 0; v < N; ++v) qg[v] = e.g[v];
+        } else {
+          static_cast<Entry<N>*>(entries_out)[pos] = e;
+        }
         if (grad) entry_vis[pos] = unsigned(i);
       }
   }
@@ -915,6 +931,8 @@ struct Entry4 {
 };
 constexpr unsigned e4_pos_w = 1u << 28;
 constexpr unsigned e4_gradient = 1u << 29;
+// k_emit<N, true> writes this layout as 2 + 2N 32-bit words
+static_assert(sizeof(Entry4<1>) == 16 && sizeof(Entry4<4>) == 40, "Entry4 layout");
 
 // HALO = false: one entry per (plane, tile) the support touches (Entry<N>,
 // for k_tile3); HALO = true: one entry per plane, keyed by the tile of the
@@ -967,7 +985,21 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
     keep[j] = false;
     const size_t i = b * bin_records + size_t(j) * bin_threads + threadIdx.x;
     if (i >= n) continue;
-    const VisRec<N>& r = recs[i];
+    // N <= 2: the whole record in registers first, so its fields load in one
+    // round instead of one round per dependent use (larger records would
+    // spill the 128 registers a 1024-thread block allows)
+    VisRec<N> r_copy;
+    if constexpr (N <= 2) {
+      constexpr int words = int(sizeof(VisRec<N>) / sizeof(unsigned long long));
+      static_assert(sizeof(VisRec<N>) % sizeof(unsigned long long) == 0,
+                    "8-byte record words");
+      const unsigned long long* src = reinterpret_cast<const unsigned long long*>(recs + i);
+      unsigned long long wds[words];
+#pragma unroll
+      for (int k = 0; k < words; ++k) wds[k] = src[k];
+      __builtin_memcpy(&r_copy, wds, sizeof(r_copy));
+    }
+    const VisRec<N>& r = N <= 2 ? r_copy : recs[i];
     const unsigned grp = r.cf_index[1];
     const unsigned cfcube = r.cf_index[0];
     const unsigned gcube = r.grid_cube;
@@ -2494,19 +2526,29 @@ hipError_t launch_tile_grid(
 hipError_t launch_count(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool update_weights, bool do_grid, CallBuffers& cb,
-  double* weights) {
+  double* weights, bool halo_entries) {
   if (n == 0) return hipSuccess;
-#define HPGX_COUNT(NP)                                                         \
+#define HPGX_COUNT(NP, H)                                                      \
   hipLaunchKernelGGL(                                                          \
-    k_count<NP>, dim3(blocks_for(n)), dim3(block), 0, s,                       \
+    (k_count<NP, H>), dim3(blocks_for(n)), dim3(block), 0, s,                  \
     static_cast<const VisRec<NP>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,    \
     int(update_weights), int(do_grid), cb.hist, weights, cb.invalid_count)
-  switch (npol) {
-  case 1: HPGX_COUNT(1); break;
-  case 2: HPGX_COUNT(2); break;
-  case 3: HPGX_COUNT(3); break;
-  case 4: HPGX_COUNT(4); break;
-  default: return hipErrorInvalidValue;
+  if (halo_entries) {
+    switch (npol) {
+    case 1: HPGX_COUNT(1, true); break;
+    case 2: HPGX_COUNT(2, true); break;
+    case 3: HPGX_COUNT(3, true); break;
+    case 4: HPGX_COUNT(4, true); break;
+    default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (npol) {
+    case 1: HPGX_COUNT(1, false); break;
+    case 2: HPGX_COUNT(2, false); break;
+    case 3: HPGX_COUNT(3, false); break;
+    case 4: HPGX_COUNT(4, false); break;
+    default: return hipErrorInvalidValue;
+    }
   }
 #undef HPGX_COUNT
   return hipGetLastError();
@@ -2517,21 +2559,31 @@ size_t entry_bytes(int npol) { return sizeof(EntryHead) + sizeof(float2) * size_
 hipError_t launch_emit(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool gvis_ready, GvisSource src, const double2* residuals,
-  CallBuffers& cb) {
+  CallBuffers& cb, bool halo_entries) {
   if (n == 0) return hipSuccess;
   const int adj = src == GvisSource::Adjoint;
-#define HPGX_EMIT(NP)                                                          \
+#define HPGX_EMIT(NP, H)                                                       \
   hipLaunchKernelGGL(                                                          \
-    k_emit<NP>, dim3(blocks_for(n)), dim3(block), 0, s,                        \
+    (k_emit<NP, H>), dim3(blocks_for(n)), dim3(block), 0, s,                   \
     static_cast<const VisRec<NP>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,    \
-    int(gvis_ready), adj, cb.gvis, residuals, cb.cursor,                       \
-    static_cast<Entry<NP>*>(cb.stream_entries), cb.entry_vis)
-  switch (npol) {
-  case 1: HPGX_EMIT(1); break;
-  case 2: HPGX_EMIT(2); break;
-  case 3: HPGX_EMIT(3); break;
-  case 4: HPGX_EMIT(4); break;
-  default: return hipErrorInvalidValue;
+    int(gvis_ready), adj, cb.gvis, residuals, cb.cursor, cb.stream_entries,    \
+    cb.entry_vis)
+  if (halo_entries) {
+    switch (npol) {
+    case 1: HPGX_EMIT(1, true); break;
+    case 2: HPGX_EMIT(2, true); break;
+    case 3: HPGX_EMIT(3, true); break;
+    case 4: HPGX_EMIT(4, true); break;
+    default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (npol) {
+    case 1: HPGX_EMIT(1, false); break;
+    case 2: HPGX_EMIT(2, false); break;
+    case 3: HPGX_EMIT(3, false); break;
+    case 4: HPGX_EMIT(4, false); break;
+    default: return hipErrorInvalidValue;
+    }
   }
 #undef HPGX_EMIT
   return hipGetLastError();
@@ -2643,16 +2695,19 @@ hipError_t launch_halo(
 hipError_t launch_tile_halo(
   hipStream_t s, int npol, int mode, int acc, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
-  int hx, int hy, void* grid) {
+  int hx, int hy, void* grid, bool global_bins) {
   if (nkeys == 0 || n == 0) return hipSuccess;
   if (hx < 0 || hy < 0 || hx >= tile || hy >= tile) return hipErrorInvalidValue;
-  const int nb = bin_blocks_for(n);
+  // global bins: one contiguous range per key, read as a single bin block
+  // whose offset-table row is the scanned offsets
+  const int nb = global_bins ? 1 : bin_blocks_for(n);
+  const unsigned* table = global_bins ? cb.offsets : cb.off_table;
   const size_t lds = halo_lds_bytes(hx, hy, nb);
   if (lds > 65536) return hipErrorInvalidValue;
 #define HPGX_TILE4(NP, MD, AC)                                                 \
   hipLaunchKernelGGL(                                                          \
     (k_tile4<NP, MD, AC>), dim3(unsigned(nkeys)), dim3(halo_block), lds, s,    \
-    geo, cf.groups, cf.taps, mu, cb.off_table, nb, block_cap,                  \
+    geo, cf.groups, cf.taps, mu, table, nb, block_cap,                         \
     static_cast<const Entry4<NP>*>(cb.stream_entries), cb.entry_vis,           \
     static_cast<const VisRec<NP>*>(cb.vis), hx, hy, grid, cb.halo, cb.halo_set)
 #define HPGX_TILE4_NP(MD, AC)                                                  \
