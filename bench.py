@@ -336,6 +336,19 @@ def main():
     results = {}
     for name, mode in [("fp32c", hpg.GridMode.FP32_COMPENSATED), ("fp64", hpg.GridMode.FP64)]:
         results[name] = run_gpu(hpg, mode, dev_records, cf, args.steps, args.warmup)
+    # the FP32 grid with FP64 tile sums instead of the two-sum tiles
+    # (HPGX_FP32_TILE is read when a state is created)
+    prev = os.environ.get("HPGX_FP32_TILE")
+    os.environ["HPGX_FP32_TILE"] = "fp64"
+    try:
+        results["fp32_fp64_tiles"] = run_gpu(hpg, hpg.GridMode.FP32_COMPENSATED, dev_records,
+                                             cf, max(1, args.steps // 4),
+                                             max(1, args.warmup // 4))
+    finally:
+        if prev is None:
+            os.environ.pop("HPGX_FP32_TILE")
+        else:
+            os.environ["HPGX_FP32_TILE"] = prev
 
     rng = np.random.default_rng(7)
     model = (rng.standard_normal((1, 1, GRID, GRID))
@@ -380,6 +393,10 @@ def main():
                       / (results["fp64"][1] * 1e-3) / 1e9 / HBM_PEAK_GBS,
                       "traffic": traffic_per_call("fp64"),
                       "finishing_ms": results["fp64"][2]},
+        "fp32_grid_fp64_tiles": {
+            "value": NUM_VIS / (results["fp32_fp64_tiles"][0] * 1e-3),
+            "device_ms_per_call": results["fp32_fp64_tiles"][1],
+            "hbm_frac": nbytes / (results["fp32_fp64_tiles"][1] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "degrid": {"vis_per_s": NUM_VIS / (degrid_ms * 1e-3),
                    "device_ms_per_call": degrid_ms,
                    "bytes_per_call": degrid_bytes(NUM_VIS),

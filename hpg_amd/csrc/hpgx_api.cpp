@@ -70,18 +70,22 @@ struct hpgx_state_s {
 
   CallBuffers cb{};
   size_t vis_capacity = 0;  // records
-  // gridding pipeline: 4 = block-local bins of one entry per record and
-  // plane, haloed LDS tiles (default; falls back to 3 when a CF support
-  // reaches the tile edge, and to 2 when the grid has more (plane, tile) keys
-  // than one workgroup's LDS histogram holds), 3 = block-local bins of one
-  // entry per touched tile, 2 = compact entries binned through global
-  // counters, 1 = per-visibility records gathered through an index
-  int gridder = 4;
+  // gridding pipeline: 5 = as 4, but FP32 grids with (hi, lo) pair tiles
+  // are added by one wave per tile quadrant with one lane per CF tap and no
+  // LDS atomics (default; 4 where the quadrant regions do not fit LDS), 4 = block-local
+  // bins of one entry per record and plane, haloed LDS tiles updated by LDS
+  // atomics (falls back to 3 when a CF support reaches the tile edge; grids
+  // with more (plane, tile) keys than one workgroup's LDS histogram holds
+  // take global bins), 3 = block-local bins of one entry per touched tile,
+  // 2 = compact entries binned through global counters, 1 = per-visibility
+  // records gathered through an index
+  int gridder = 5;
   size_t nkeys = 0;
   size_t pass_blocks = 0;  // bin blocks per pass of the block-local pipelines
-  // tile accumulation of FP32 grids in the haloed-tile gridder: 2 = FP32
-  // hi/lo pairs updated by two-sum under compare-and-swap (default), 1 = FP32
-  // two-sum through returning FP32 LDS atomics, 0 = FP64 tile sums
+  // tile accumulation of FP32 grids in the haloed-tile gridders: 2 = FP32
+  // hi/lo pairs updated by two-sum (default; under compare-and-swap in
+  // gridder 4), 1 = FP32 two-sum through returning FP32 LDS atomics
+  // (gridder 4 only), 0 = FP64 tile sums
   int fp32_tile = 2;
 
   Plan plan_z2z, plan_c2c;
@@ -203,7 +207,7 @@ int ensure_cf_capacity(hpgx_state_t st, size_t taps, size_t ws) {
 // block-local bins: the pipelines of choice when one workgroup's LDS holds
 // a histogram over all (plane, tile) keys
 bool use_block_local(hpgx_state_t st) {
-  return (st->gridder == 3 || st->gridder == 4)
+  return (st->gridder >= 3 && st->gridder <= 5)
     && st->nkeys <= size_t(hpgx::bin_keys_limit());
 }
 
@@ -213,7 +217,7 @@ int halo_y(hpgx_state_t st) { return std::max(1, st->cf.max_sy) - 1; }
 
 // haloed tiles: supports narrower than a tile, tile + halo within LDS
 bool halo_ok(hpgx_state_t st) {
-  if (st->gridder != 4) return false;
+  if (st->gridder != 4 && st->gridder != 5) return false;
   const int hx = halo_x(st), hy = halo_y(st);
   return hx < hpgx::halo_tile_limit() && hy < hpgx::halo_tile_limit()
     && hpgx::halo_lds_bytes(hx, hy, 1) <= 65536;
@@ -228,11 +232,23 @@ bool use_halo_global(hpgx_state_t st) {
   return halo_ok(st) && st->nkeys > size_t(hpgx::bin_keys_limit());
 }
 
+// the quadrant kernel (gridder 5) for FP32 grids with (hi, lo) pair tiles:
+// its regions and tables fit LDS and CF tap indexes fit 32 bits.  FP64 tile
+// sums stay with k_tile4, whose non-returning FP64 LDS atomics need no
+// read-modify-write per tap.
+bool use_quad(hpgx_state_t st) {
+  return st->gridder == 5 && (use_halo(st) || use_halo_global(st))
+    && st->mode != HPGX_GRID_FP64 && st->fp32_tile == 2
+    && hpgx::quad_lds_bytes(halo_x(st), halo_y(st), int(st->npol)) <= 65536
+    && st->taps_capacity < (size_t(1) << 32);
+}
+
 // bin blocks per haloed-tile pass: the per-block segment tables share the
 // workgroup's 64 KiB of LDS with the tile
 size_t halo_bins_limit(hpgx_state_t st) {
   size_t nb = size_t(hpgx::tile_bins_limit());
   while (nb > 1 && hpgx::halo_lds_bytes(halo_x(st), halo_y(st), int(nb)) > 65536) nb /= 2;
+  if (use_quad(st)) nb = std::min(nb, size_t(hpgx::quad_bins_limit()));
   return nb;
 }
 
@@ -627,7 +643,7 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
       if (halo)
         HPGX_HIP(st, hpgx::launch_tile_halo(
                        st->stream, np, mode, st->fp32_tile, m, st->nkeys, geo, st->cf, st->mu, c,
-                       block_cap, halo_x(st), halo_y(st), st->grid));
+                       block_cap, halo_x(st), halo_y(st), st->grid, false, use_quad(st)));
       else
         HPGX_HIP(st, hpgx::launch_tile_blocal(
                        st->stream, np, mode, m, st->nkeys, geo, st->cf, st->mu, c,
@@ -656,7 +672,7 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
       if (halo)
         HPGX_HIP(st, hpgx::launch_tile_halo(
                        st->stream, np, mode, st->fp32_tile, m, st->nkeys, geo, st->cf, st->mu,
-                       c, 0, halo_x(st), halo_y(st), st->grid, true));
+                       c, 0, halo_x(st), halo_y(st), st->grid, true, use_quad(st)));
       else
         HPGX_HIP(st, hpgx::launch_tile_stream(
                        st->stream, np, mode, st->nkeys, geo, st->cf, st->mu, c, st->grid));
@@ -773,7 +789,7 @@ int hpgx_create(
   }
   if (const char* g = std::getenv("HPGX_GRIDDER")) {
     const int v = std::atoi(g);
-    st->gridder = (v >= 1 && v <= 4) ? v : 4;
+    st->gridder = (v >= 1 && v <= 5) ? v : 5;
   }
   // Mueller index matrices padded to 4x4 with -1
   // (copy_mueller_indexes, src/hpg_runtime.hpp:1590-1614)

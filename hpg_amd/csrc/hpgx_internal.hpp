@@ -156,12 +156,16 @@ size_t halo_entry_bytes(int npol);
 size_t halo_pixels_per_key(int hx, int hy);
 int halo_tile_limit();
 size_t halo_lds_bytes(int hx, int hy, int nbins);
+// LDS of the quadrant kernel (k_quad) and its bin blocks per pass
+size_t quad_lds_bytes(int hx, int hy, int npol);
+int quad_bins_limit();
 // acc (FP32 grids): 0 FP64 tile sums, 1 FP32 two-sum via returning atomics,
-// 2 FP32 two-sum via compare-and-swap of packed (hi, lo) pairs
+// 2 FP32 two-sum of (hi, lo) pairs (compare-and-swap in k_tile4, plain
+// read-modify-write in k_quad).  quad: k_quad (acc 0 or 2) instead of k_tile4
 hipError_t launch_tile_halo(
   hipStream_t s, int npol, int mode, int acc, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
-  int hx, int hy, void* grid, bool global_bins = false);
+  int hx, int hy, void* grid, bool global_bins = false, bool quad = false);
 hipError_t launch_tile_blocal(
   hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,

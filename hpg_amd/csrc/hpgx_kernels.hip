@@ -1842,6 +1842,489 @@ __global__ void k_halo(
   *gp = g;
 }
 
+// ---- gridding: haloed tiles, one wave per tile quadrant, one lane per tap --
+//
+// k_quad accumulates the keys and entries of k_tile4 into an FP32 grid,
+// (hi, lo) pairs per pixel updated by two-sum as in k_tile4's
+// compare-and-swap scheme, without LDS atomics.
+// The workgroup of a (plane, tile) key has four waves; wave w owns quadrant
+// (w & 1, w >> 1) of the 32 x 32 tile -- the entries whose support corner
+// lies in that 16 x 16 quadrant -- and a private LDS region of
+// (16 + hx) x (16 + hy) pixels, which holds those supports whole.  The wave
+// adds one entry at a time, one lane per CF tap: an entry's taps are
+// distinct pixels, a wave's LDS operations execute in order and no other wave
+// writes the region, so each tap is a plain 16-byte read-modify-write.  Per
+// round each thread prepares one entry (CF slice, values, region pixel) and
+// the round's entries are sorted by quadrant into LDS.  The flush sums the
+// four regions over the tile and its halo and writes both as k_tile4 does,
+// so k_halo folds the halos unchanged.
+constexpr int quad_block = 4 * wave;
+constexpr int quad_edge = tile / 2;
+constexpr int quad_bins_max = 256;  // bin blocks per pass (LDS segment tables)
+// entries whose first CF taps are loaded ahead of their turn
+template <int N>
+constexpr int quad_depth() { return N == 1 ? 4 : 2; }
+
+// an entry prepared for its quadrant's wave; gc = (pos_w ? -1 : 1) * g with
+// re and im swapped, so that cmul(conj_w(c), g) = (c.x g.x - c.y gc.x,
+// c.x g.y + c.y gc.y) with the reference's roundings.  Entries with a phase
+// gradient do not use gc: gc[0].x holds the bits of their entry index.
+template <int N>
+struct QuadRec {
+  unsigned w0;        // region pixel | sx << 12 | sy << 18 | use << 24 | flags
+  unsigned slice[N];  // first tap of each polarization's CF slice
+  float2 g[N];
+  float2 gc[N];
+};
+constexpr unsigned quad_gradient = 1u << 28;
+constexpr unsigned quad_pos_w = 1u << 29;
+template <int N>
+constexpr int quad_round() { return N == 1 ? quad_block : quad_block / 2; }
+
+__device__ __forceinline__ unsigned uniform_u(unsigned v) {
+  return unsigned(__builtin_amdgcn_readfirstlane(int(v)));
+}
+
+// adds the tap value h + l (FP32 high and low parts) to the (hi, lo) pairs
+// of a region pixel: hi' = fl(hi + h), lo' = fl(lo + (err + l)), err the
+// exact rounding error of the first sum (two-sum)
+template <int N>
+__device__ __forceinline__ void quad_add(
+  float4* region, int px, float h_r, float h_i, float l_r, float l_i) {
+  float4 a = region[px];
+  float sum, err;
+  two_sum(a.x, h_r, sum, err);
+  a.x = sum;
+  a.z = N > 1 ? a.z + (err + l_r) : a.z + err;
+  two_sum(a.y, h_i, sum, err);
+  a.y = sum;
+  a.w = N > 1 ? a.w + (err + l_i) : a.w + err;
+  region[px] = a;
+}
+
+// an entry with a phase gradient: its CF taps times the phase screen
+// exp(i phi(X, Y)) (FP64 phase, the reference's cphase) added by one wave,
+// one lane per tap.  Not inlined: the FP64 sine and cosine would otherwise
+// raise the register count of k_quad's pipelined loop.
+template <int N>
+__device__ __noinline__ void quad_gradient_entry(
+  const QuadRec<N>& p, unsigned ei, const Geometry& geo, const float2* __restrict__ taps,
+  const Entry4<N>* __restrict__ entries, const unsigned* __restrict__ entry_vis,
+  const VisRec<N>* __restrict__ recs, int QX, float4* region) {
+  const int lane = int(threadIdx.x % wave);
+  const unsigned w0 = uniform_u(p.w0);
+  const int base = int(w0 & 4095u);
+  const int sx = int((w0 >> 12) & 63u);
+  const int sy = int((w0 >> 18) & 63u);
+  const unsigned use = (w0 >> 24) & 15u;
+  const float imf = (w0 & quad_pos_w) ? -1.0f : 1.0f;
+  ei = uniform_u(ei);
+  const unsigned pos = entries[ei].h.pos;
+  const int fxi = int((pos >> 12) & 255u);
+  const int fyi = int((pos >> 20) & 255u);
+  const unsigned ri = entry_vis[ei];
+  const double gxr = recs[ri].grad[0];
+  const double gyr = recs[ri].grad[1];
+  const double phi0x = -gxr * ((sx / 2) * geo.os - (fxi - geo.F));
+  const double phi0y = -gyr * ((sy / 2) * geo.os - (fyi - geo.F));
+  const double dphix = gxr * geo.os;
+  const double dphiy = gyr * geo.os;
+  for (int t0 = 0; t0 < sx * sy; t0 += wave) {
+    const int tp = t0 + lane;
+    if (tp >= sx * sy) continue;
+    const int Y = tp / sx;
+    const int X = tp - Y * sx;
+    const float2 scr = narrow(cphase((phi0x + X * dphix) + (phi0y + Y * dphiy)));
+    double sr = 0.0, si = 0.0;
+    float h_r = 0.0f, h_i = 0.0f, l_r = 0.0f, l_i = 0.0f;
+    for (int v = 0; v < N; ++v) {
+      if (!((use >> v) & 1u)) continue;
+      float2 cc = taps[uniform_u(p.slice[v]) + size_t(tp)];
+      cc.y *= imf;
+      const float2 pr = cmul(cmul(cc, scr), p.g[v]);
+      if constexpr (N == 1) {
+        h_r = pr.x;
+        h_i = pr.y;
+      }
+      sr += double(pr.x);
+      si += double(pr.y);
+    }
+    if constexpr (N > 1) {
+      h_r = float(sr);
+      h_i = float(si);
+      l_r = float(sr - double(h_r));
+      l_i = float(si - double(h_i));
+    }
+    quad_add<N>(region, base + Y * QX + X, h_r, h_i, l_r, l_i);
+  }
+}
+
+// ONE: every support fits one step (sx, sy <= 8); the pipelined loop is
+// then free of branches around vector memory loads, so each entry waits for
+// its own CF taps only and not for the loads issued after them
+template <int N, bool ONE>
+__global__ __launch_bounds__(quad_block) void k_quad(
+  Geometry geo, const GroupInfo* __restrict__ groups,
+  const float2* __restrict__ taps, Mueller mu, const unsigned* __restrict__ off_table,
+  int nbins, size_t block_cap, const Entry4<N>* __restrict__ entries,
+  const unsigned* __restrict__ entry_vis, const VisRec<N>* __restrict__ recs,
+  int hx, int hy, void* __restrict__ grid, double2* __restrict__ halo,
+  unsigned* __restrict__ halo_set) {
+  constexpr int RB = quad_round<N>();
+  // quadrant regions, 16 bytes per pixel: FP32 (hi_re, hi_im, lo_re, lo_im)
+  extern __shared__ float4 regions[];
+  __shared__ QuadRec<N> list[RB];
+  __shared__ unsigned seg_pref[quad_bins_max + 1];
+  __shared__ unsigned seg_beg[quad_bins_max];
+  __shared__ unsigned part[quad_block / wave + 1];
+  __shared__ unsigned qcnt[5];  // entries per quadrant, then a gradient flag
+  unsigned* qgrad = qcnt + 4;
+
+  const int QX = quad_edge + hx;
+  const int QY = quad_edge + hy;
+  const int Q = QX * QY;
+  const unsigned key = blockIdx.x;
+  const int nkeys = geo.ntx * geo.nty * geo.nr * geo.nc;
+  for (int bb = threadIdx.x; bb < nbins; bb += quad_block) {
+    const unsigned* tab = off_table + size_t(bb) * (nkeys + 1);
+    const unsigned s0 = tab[key], s1 = tab[key + 1];
+    seg_pref[bb] = s1 - s0;
+    seg_beg[bb] = s0;
+  }
+  __syncthreads();
+  const unsigned total = block_exclusive_scan(seg_pref, nbins, part);
+  if (total == 0) {
+    if (threadIdx.x == 0) halo_set[key] = 0u;
+    return;
+  }
+  if (threadIdx.x == 0) seg_pref[nbins] = total;
+
+  const int ntiles = geo.ntx * geo.nty;
+  const unsigned plane = key / ntiles;
+  const int t = int(key % ntiles);
+  const int x0 = (t % geo.ntx) * tile;
+  const int y0 = (t / geo.ntx) * tile;
+  const int gpol = int(plane % geo.nr);
+
+  for (int k = threadIdx.x; k < 4 * Q; k += quad_block)
+    regions[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  __syncthreads();
+
+  const unsigned tid = threadIdx.x;
+  const int w = int(tid) / wave;
+  const int lane = int(tid) % wave;
+  float4* reg32 = regions + size_t(w) * Q;
+
+  auto locate = [&](unsigned q) {
+    int lo = 0, hi = nbins;  // seg_pref[lo] <= q < seg_pref[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (seg_pref[mid] <= q) lo = mid; else hi = mid;
+    }
+    return unsigned(lo) * unsigned(block_cap) + seg_beg[lo] + (q - seg_pref[lo]);
+  };
+  // this thread's raw entry of the next round
+  Entry4<N> ne{};
+  unsigned nidx = 0;
+  auto load_round = [&](unsigned r0) {
+    if (tid < unsigned(RB) && r0 + tid < total) {
+      nidx = locate(r0 + tid);
+      ne = entries[nidx];
+    }
+  };
+  load_round(0);
+
+  // Lane layout of an entry's taps, step st of its steps: supports up to 8
+  // wide put tap (X, Y) = (lane & 7, lane >> 3 + 8 st) on this lane, so that
+  // the rows of one 16-lane LDS phase use distinct banks when the region row
+  // is 8 modulo 16 pixels; wider supports take the taps t = 64 st + lane
+  // contiguously, Y = t / sx from a float reciprocal (exact here: t < 4096
+  // and (t + 1/2) / sx stays 1/(2 sx) > 0.0079 away from an integer, far
+  // beyond the 4e-4 error of the product).
+  const int lane_x8 = lane & 7;
+  const int lane_y8 = lane >> 3;
+  auto tap_xy = [&](int sx, float rsx, int st, int& X, int& Y) {
+    if (ONE || sx <= 8) {
+      X = lane_x8;
+      Y = lane_y8 + st * (wave / 8);
+    } else {
+      const int tp = st * wave + lane;
+      Y = int((float(tp) + 0.5f) * rsx);
+      X = tp - Y * sx;
+    }
+  };
+  auto num_steps = [](int sx, int sy) {
+    return sx <= 8 ? (sy + wave / 8 - 1) / (wave / 8) : (sx * sy + wave - 1) / wave;
+  };
+
+  // an entry in flight: its prepared values and its first step's CF taps
+  struct Slot {
+    unsigned i, w0;
+    const float2* slice[N];
+    float2 c[N];
+  };
+  // The first step's taps load unconditionally, lanes outside the support
+  // reading tap 0 (every slice index points at a whole slice): a load under
+  // a condition would need its register cleared first, which waits for all
+  // loads in flight.
+  auto fetch = [&](unsigned i, Slot& s) {
+    const QuadRec<N>& p = list[i];
+    const unsigned w0v = p.w0;
+    unsigned slv[N];
+#pragma unroll
+    for (int v = 0; v < N; ++v) slv[v] = p.slice[v];
+    s.i = i;
+    s.w0 = uniform_u(w0v);
+    const int sx = int((s.w0 >> 12) & 63u);
+    const int sy = int((s.w0 >> 18) & 63u);
+    int X, Y;
+    tap_xy(sx, __builtin_amdgcn_rcpf(float(sx)), 0, X, Y);
+    const bool ok = unsigned(X) < unsigned(sx) && unsigned(Y) < unsigned(sy);
+    const int tp = ok ? Y * sx + X : 0;
+#pragma unroll
+    for (int v = 0; v < N; ++v) {
+      s.slice[v] = taps + uniform_u(slv[v]);
+      s.c[v] = s.slice[v][tp];
+    }
+  };
+  auto add_tap = [&](int px, float h_r, float h_i, float l_r, float l_i) {
+    quad_add<N>(reg32, px, h_r, h_i, l_r, l_i);
+  };
+  const int lane_px = lane_y8 * QX + lane_x8;
+  // an entry of supports up to 8 x 8 (ONE): a single step, no branch when
+  // the entry is live; `live` false (past the wave's entries) adds nothing
+  auto process_one = [&](Slot& s, bool live) {
+    const int sx = int((s.w0 >> 12) & 63u);
+    const int sy = int((s.w0 >> 18) & 63u);
+    const unsigned use = (s.w0 >> 24) & 15u;
+    live = live && !(s.w0 & quad_gradient) && use != 0u;
+    const QuadRec<N>& p = list[s.i];
+    if (live && lane_x8 < sx && lane_y8 < sy) {
+      const int px = int(s.w0 & 4095u) + lane_px;
+      if constexpr (N == 1) {
+        const float2 c = s.c[0];
+        const float2 g = p.g[0], gc = p.gc[0];
+        const float pr = c.x * g.x - c.y * gc.x;
+        const float pi = c.x * g.y + c.y * gc.y;
+        add_tap(px, pr, pi, 0.0f, 0.0f);
+      } else {
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int v = 0; v < N; ++v) {
+          if (!((use >> v) & 1u)) continue;
+          const float2 c = s.c[v];
+          const float2 g = p.g[v], gc = p.gc[v];
+          sr += double(c.x * g.x - c.y * gc.x);
+          si += double(c.x * g.y + c.y * gc.y);
+        }
+        const float h_r = float(sr), h_i = float(si);
+        add_tap(px, h_r, h_i, float(sr - double(h_r)), float(si - double(h_i)));
+      }
+    }
+  };
+  auto process = [&](Slot& s) {
+    if ((s.w0 & quad_gradient) || !((s.w0 >> 24) & 15u)) return;
+    const int base = int(s.w0 & 4095u);
+    const int sx = int((s.w0 >> 12) & 63u);
+    const int sy = int((s.w0 >> 18) & 63u);
+    const unsigned use = (s.w0 >> 24) & 15u;
+    const float rsx = __builtin_amdgcn_rcpf(float(sx));
+    const int steps = num_steps(sx, sy);
+    for (int st = 0; st < steps; ++st) {
+      int X, Y;
+      tap_xy(sx, rsx, st, X, Y);
+      if (st > 0) {
+        const bool okn = unsigned(X) < unsigned(sx) && unsigned(Y) < unsigned(sy);
+        const int tp = okn ? Y * sx + X : 0;
+#pragma unroll
+        for (int v = 0; v < N; ++v) s.c[v] = s.slice[v][tp];
+      }
+      if (!(unsigned(X) < unsigned(sx) && unsigned(Y) < unsigned(sy))) continue;
+      // FP32 products c * g (conjugated c for w > 0), their FP64 sum over
+      // the polarizations
+      const QuadRec<N>& p = list[s.i];
+      if constexpr (N == 1) {
+        const float2 c = s.c[0];
+        const float2 g = p.g[0], gc = p.gc[0];
+        const float pr = c.x * g.x - c.y * gc.x;
+        const float pi = c.x * g.y + c.y * gc.y;
+        add_tap(base + Y * QX + X, pr, pi, 0.0f, 0.0f);
+      } else {
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int v = 0; v < N; ++v) {
+          if (!((use >> v) & 1u)) continue;
+          const float2 c = s.c[v];
+          const float2 g = p.g[v], gc = p.gc[v];
+          sr += double(c.x * g.x - c.y * gc.x);
+          si += double(c.x * g.y + c.y * gc.y);
+        }
+        const float h_r = float(sr), h_i = float(si);
+        add_tap(base + Y * QX + X, h_r, h_i, float(sr - double(h_r)), float(si - double(h_i)));
+      }
+    }
+  };
+  for (unsigned r0 = 0; r0 < total; r0 += RB) {
+    const unsigned rn = min(unsigned(RB), total - r0);
+    const bool mine = tid < rn;
+    const Entry4<N> e = ne;
+    const unsigned idx = nidx;
+    if (tid < 5) qcnt[tid] = 0u;
+    __syncthreads();  // the previous round's entries are all added
+    QuadRec<N> rec{};
+    int qd = -1;
+    if (mine) {
+      const unsigned pos = e.h.pos;
+      const int lx = int(pos & 63u);
+      const int ly = int((pos >> 6) & 63u);
+      const int fxi = int((pos >> 12) & 255u);
+      const int fyi = int((pos >> 20) & 255u);
+      const bool pos_w = pos & e4_pos_w;
+      const bool grad = pos & e4_gradient;
+      const int grp = int(e.h.gpk & 0xffffu);
+      const int cfcube = int(e.h.gpk >> 16);
+      const GroupInfo g = groups[grp];
+      const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
+      const long long slice_taps = (long long)g.sx * g.sy;
+      const float imf = pos_w ? -1.0f : 1.0f;
+      unsigned use = 0;
+#pragma unroll
+      for (int v = 0; v < N; ++v) {
+        const int m = row[v];
+        const bool u = m >= 0 && m < g.nmueller;
+        use |= u ? 1u << v : 0u;
+        rec.slice[v] = unsigned(
+          g.tap_offset
+          + (((long long)(cfcube * g.nmueller + (u ? m : 0)) * geo.nf + fxi) * geo.nf + fyi)
+            * slice_taps);
+        rec.g[v] = e.g[v];
+        rec.gc[v] = make_float2(imf * e.g[v].y, imf * e.g[v].x);
+      }
+      const int qx = lx >= quad_edge ? 1 : 0;
+      const int qy = ly >= quad_edge ? 1 : 0;
+      qd = qx | (qy << 1);
+      const unsigned base = unsigned((ly - qy * quad_edge) * QX + (lx - qx * quad_edge));
+      rec.w0 = base | (unsigned(g.sx) << 12) | (unsigned(g.sy) << 18) | (use << 24)
+        | (grad ? quad_gradient : 0u) | (pos_w ? quad_pos_w : 0u);
+      if (grad) rec.gc[0].x = __uint_as_float(idx);
+    }
+    // counting sort by quadrant: one LDS atomic per wave and quadrant
+    if (__ballot(mine && (rec.w0 & quad_gradient)) != 0ull && lane == 0)
+      __hip_atomic_fetch_or(qgrad, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    unsigned slot = 0;
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const unsigned long long m = __ballot(qd == k);
+      unsigned wbase = 0;
+      if (lane == 0 && m != 0ull)
+        wbase = __hip_atomic_fetch_add(qcnt + k, unsigned(__popcll(m)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+      wbase = __shfl(wbase, 0, wave);
+      if (qd == k) slot = wbase + unsigned(__popcll(m & below));
+    }
+    __syncthreads();
+    const unsigned c0 = qcnt[0], c1 = qcnt[1], c2 = qcnt[2], c3 = qcnt[3];
+    const bool any_gradient = *qgrad != 0u;
+    if (mine) {
+      const unsigned off = (qd > 0 ? c0 : 0u) + (qd > 1 ? c1 : 0u) + (qd > 2 ? c2 : 0u) + slot;
+      list[off] = rec;
+    }
+    load_round(r0 + RB);
+    __syncthreads();
+
+    // this wave's entries of the round, D in flight
+    const unsigned beg = (w > 0 ? c0 : 0u) + (w > 1 ? c1 : 0u) + (w > 2 ? c2 : 0u);
+    const unsigned end = beg + (w == 0 ? c0 : w == 1 ? c1 : w == 2 ? c2 : c3);
+    constexpr int D = quad_depth<N>();
+    Slot sl[D];
+    if constexpr (ONE) {
+      // clamped fetches past the end reload the last entry and are not added
+      if (beg < end) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) fetch(min(beg + k, end - 1), sl[k]);
+        for (unsigned i = beg; i < end; i += D) {
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            process_one(sl[k], i + k < end);
+            fetch(min(i + k + D, end - 1), sl[k]);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+        if (beg + k < end) fetch(beg + k, sl[k]);
+      for (unsigned i = beg; i < end; i += D) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          if (i + k < end) {
+            process(sl[k]);
+            if (i + k + D < end) fetch(i + k + D, sl[k]);
+          }
+        }
+      }
+    }
+    if (any_gradient)
+#pragma nounroll
+      for (unsigned i = beg; i < end; ++i)
+        if (uniform_u(list[i].w0) & quad_gradient)
+          quad_gradient_entry<N>(list[i], __float_as_uint(list[i].gc[0].x), geo, taps, entries,
+                                 entry_vis, recs, QX, reg32);
+  }
+  __syncthreads();
+
+  // flush: the sum of the quadrant regions over the tile and its halo
+  const int RX = tile + hx;
+  const int RY = tile + hy;
+  const int R = RX * RY;
+  const int hs = halo_pixels(hx, hy);
+  double2* hk = halo + size_t(key) * hs;
+  for (int k = threadIdx.x; k < R; k += quad_block) {
+    const int ry = k / RX;
+    const int rx = k - ry * RX;
+    // (hi, lo) of the pixel: the quadrants' pairs combined by two-sum
+    float fr = 0.0f, fi = 0.0f, flr = 0.0f, fli = 0.0f;
+    bool first = true;
+#pragma unroll
+    for (int qd = 0; qd < 4; ++qd) {
+      const int ux = rx - (qd & 1) * quad_edge;
+      const int uy = ry - (qd >> 1) * quad_edge;
+      if (ux < 0 || uy < 0 || ux >= QX || uy >= QY) continue;
+      const float4 a = regions[qd * Q + uy * QX + ux];
+      if (first) {
+        fr = a.x;
+        fi = a.y;
+        flr = a.z;
+        fli = a.w;
+      } else {
+        float sum, err;
+        two_sum(fr, a.x, sum, err);
+        fr = sum;
+        flr = flr + (err + a.z);
+        two_sum(fi, a.y, sum, err);
+        fi = sum;
+        fli = fli + (err + a.w);
+      }
+      first = false;
+    }
+    if (rx < tile && ry < tile) {
+      const int xx = x0 + rx, yy = y0 + ry;
+      if (xx >= geo.nx || yy >= geo.ny) continue;
+      float2* gp = static_cast<float2*>(grid) + (size_t(plane) * geo.ny + yy) * geo.nx + xx;
+      const float2 g0 = *gp;
+      float sr, er, si, ei;
+      two_sum(g0.x, fr, sr, er);
+      two_sum(g0.y, fi, si, ei);
+      *gp = make_float2(sr + (er + flr), si + (ei + fli));
+    } else {
+      const int h = ry < tile ? ry * hx + (rx - tile) : tile * hx + (ry - tile) * RX + rx;
+      hk[h] = make_double2(double(fr) + double(flr), double(fi) + double(fli));
+    }
+  }
+  if (threadIdx.x == 0) halo_set[key] = 1u;
+}
+
 // ---- degridding --------------------------------------------------------
 
 template <int N>
@@ -2692,16 +3175,56 @@ hipError_t launch_halo(
   return hipGetLastError();
 }
 
+size_t quad_lds_bytes(int hx, int hy, int npol) {
+  const size_t regions = 4 * size_t(quad_edge + hx) * (quad_edge + hy) * sizeof(float4);
+  size_t rec = 0, rb = 0;
+  switch (npol) {
+  case 1: rec = sizeof(QuadRec<1>); rb = quad_round<1>(); break;
+  case 2: rec = sizeof(QuadRec<2>); rb = quad_round<2>(); break;
+  case 3: rec = sizeof(QuadRec<3>); rb = quad_round<3>(); break;
+  default: rec = sizeof(QuadRec<4>); rb = quad_round<4>(); break;
+  }
+  return regions + rb * rec
+    + sizeof(unsigned) * (2 * size_t(quad_bins_max) + 1 + quad_block / wave + 1 + 5);
+}
+int quad_bins_limit() { return quad_bins_max; }
+
 hipError_t launch_tile_halo(
   hipStream_t s, int npol, int mode, int acc, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
-  int hx, int hy, void* grid, bool global_bins) {
+  int hx, int hy, void* grid, bool global_bins, bool quad) {
   if (nkeys == 0 || n == 0) return hipSuccess;
   if (hx < 0 || hy < 0 || hx >= tile || hy >= tile) return hipErrorInvalidValue;
   // global bins: one contiguous range per key, read as a single bin block
   // whose offset-table row is the scanned offsets
   const int nb = global_bins ? 1 : bin_blocks_for(n);
   const unsigned* table = global_bins ? cb.offsets : cb.off_table;
+  if (quad) {
+    if (nb > quad_bins_max || quad_lds_bytes(hx, hy, npol) > 65536 || mode != 1
+        || acc != acc_f32_cas)
+      return hipErrorInvalidValue;
+    const size_t dyn = 4 * size_t(quad_edge + hx) * (quad_edge + hy) * sizeof(float4);
+    // every support within one lane step (hx, hy = largest support - 1)
+    const bool one = hx < 8 && hy < 8;
+#define HPGX_QUAD(NP)                                                          \
+  hipLaunchKernelGGL(                                                          \
+    one ? (k_quad<NP, true>) : (k_quad<NP, false>),                            \
+    dim3(unsigned(nkeys)), dim3(quad_block), dyn, s,                           \
+    geo, cf.groups, cf.taps, mu, table, nb, block_cap,                         \
+    static_cast<const Entry4<NP>*>(cb.stream_entries), cb.entry_vis,           \
+    static_cast<const VisRec<NP>*>(cb.vis), hx, hy, grid, cb.halo, cb.halo_set)
+    switch (npol) {
+    case 1: HPGX_QUAD(1); break;
+    case 2: HPGX_QUAD(2); break;
+    case 3: HPGX_QUAD(3); break;
+    case 4: HPGX_QUAD(4); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef HPGX_QUAD
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_halo(s, mode, nkeys, geo, cb, hx, hy, grid);
+  }
   const size_t lds = halo_lds_bytes(hx, hy, nb);
   if (lds > 65536) return hipErrorInvalidValue;
 #define HPGX_TILE4(NP, MD, AC)                                                 \

@@ -135,15 +135,17 @@ def test_random_vs_oracle_fp32_compensated(hpg, oracle_mod, case):
     np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
 
 
-@pytest.mark.parametrize("gridder", [1, 2, 3, 4])
+@pytest.mark.parametrize("gridder", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("case", [0, 2, 3, 4])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_each_gridder_vs_oracle(hpg, oracle_mod, monkeypatch, gridder, case, mode):
     """Every gridding pipeline (HPGX_GRIDDER, read when a state is created:
     1 indexed records, 2 globally binned entries, 3 block-local bins with
-    clipped tiles, 4 block-local bins with haloed tiles -- which falls back
-    to 3 for case 3, whose 35-pixel support exceeds a tile) against the
-    oracle, with phase gradients on a third of the records and some records
+    clipped tiles, 4 block-local bins with haloed tiles updated by LDS
+    atomics -- which falls back to 3 for case 3, whose 35-pixel support
+    exceeds a tile -- and 5 the haloed tiles added per quadrant wave, one
+    lane per tap, falling back to 4 where the quadrant regions do not fit
+    LDS) against the oracle, with phase gradients on a third of the records and some records
     off the grid.  FP64 mode:
     1e-6 of max|grid| (gradient screens); FP32-compensated: e_max, e_rms
     <= 1e-6 per plane."""
@@ -170,22 +172,25 @@ def test_each_gridder_vs_oracle(hpg, oracle_mod, monkeypatch, gridder, case, mod
     np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
 
 
+@pytest.mark.parametrize("gridder", [4, 5])
 @pytest.mark.parametrize("scheme", ["cas", "rtn", "fp64"])
 @pytest.mark.parametrize("case", range(len(CASES)))
-def test_fp32_tile_schemes(hpg, oracle_mod, monkeypatch, scheme, case):
+def test_fp32_tile_schemes(hpg, oracle_mod, monkeypatch, gridder, scheme, case):
     """Each tile accumulation of the FP32 grid (HPGX_FP32_TILE: FP32 two-sum
-    pairs under compare-and-swap, FP32 two-sum through returning atomics, FP64
+    pairs -- under compare-and-swap in gridder 4, plain read-modify-write in
+    gridder 5 --, FP32 two-sum through returning atomics (gridder 4), FP64
     tile sums) against the FP64 oracle: e_max, e_rms <= 1e-6 per plane.  The
     records are dense (3000 on grids of 40-70 pixels), so pixels collect
     hundreds of contributions and lanes collide on pixels."""
     from hpg_amd import workloads
+    monkeypatch.setenv("HPGX_GRIDDER", str(gridder))
     monkeypatch.setenv("HPGX_FP32_TILE", scheme)
     npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[case]
     cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=60 + case)
     gs, o = make_pair(hpg, oracle_mod, gsz, (0.9, 1.1), M, cf, 1, Mc)
     rec = workloads.random_visibilities(3000, npol, gsz, (0.9, 1.1), cf, seed=70 + case,
                                         phase=True)
-    print(f"scheme {scheme} case {case}", flush=True)
+    print(f"gridder {gridder} scheme {scheme} case {case}", flush=True)
     gs.grid_visibilities(hpg.Device.Serial, rec, True)
     o.grid_visibilities(oracle_view(rec, oracle_mod), True)
     got, exp = gs.grid_values(), o.grid_values()
@@ -197,21 +202,23 @@ def test_fp32_tile_schemes(hpg, oracle_mod, monkeypatch, scheme, case):
             assert e_max <= 1e-6 and e_rms <= 1e-6, (c, r, e_max, e_rms)
 
 
-@pytest.mark.parametrize("radius", [(15, 2), (2, 15), (16, 1), (0, 0)])
+@pytest.mark.parametrize("gridder", [4, 5])
+@pytest.mark.parametrize("radius", [(15, 2), (2, 15), (16, 1), (6, 6), (0, 0)])
 @pytest.mark.parametrize("mode", [0, 1])
-def test_support_at_tile_edge(hpg, oracle_mod, radius, mode):
+def test_support_at_tile_edge(hpg, oracle_mod, monkeypatch, gridder, radius, mode):
     """Largest supports the haloed tiles take (31 pixels on one axis), the
-    first they hand to the clipped tiles (33), and a 1x1 support (no halo)
-    on a grid whose edges are not tile multiples; records cover the grid up
-    to its edges.  No phase gradients: FP64 grids to 1e-12 of max|grid|,
-    FP32-compensated to 1e-6."""
+    first they hand to the clipped tiles (33), 13 x 13 (the quadrant kernel's
+    largest regions), and a 1x1 support (no halo) on a grid whose edges are
+    not tile multiples; records cover the grid up to its edges.  No phase
+    gradients: FP64 grids to 1e-12 of max|grid|, FP32-compensated to 1e-6."""
     from hpg_amd import workloads
+    monkeypatch.setenv("HPGX_GRIDDER", str(gridder))
     gsz = (97, 75, 1, 1)
     cf = workloads.random_cf(4, [radius], seed=7)
     gs, o = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), [[0]], cf, mode)
     rec = workloads.random_visibilities(2500, 1, gsz, (1.0, 1.0), cf, seed=8, phase=True,
                                         margin=-2)
-    print(f"support radius {radius} mode {mode}", flush=True)
+    print(f"gridder {gridder} support radius {radius} mode {mode}", flush=True)
     gs.grid_visibilities(hpg.Device.Serial, rec, True)
     o.grid_visibilities(oracle_view(rec, oracle_mod), True)
     got, exp = gs.grid_values(), o.grid_values()
@@ -239,12 +246,12 @@ def test_serialization_and_batches(hpg, oracle_mod):
     np.testing.assert_allclose(gs1.grid_weights(), gs2.grid_weights(), rtol=1e-13)
 
 
-@pytest.mark.parametrize("gridder", [3, 4])
+@pytest.mark.parametrize("gridder", [3, 4, 5])
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_degrid_and_residual_vs_oracle(hpg, oracle_mod, monkeypatch, gridder, case):
     """Predicted values, residuals and the gridded residuals against the
     oracle: 1e-6 relative (FP32 outputs of FP64 sums), grids to 1e-5.  With
-    the haloed-tile gridder (4) and supports below 32 pixels the degridder
+    the haloed-tile gridders (4, 5) and supports below 32 pixels the degridder
     is binned (model tiles in LDS, per-record sums, then the finishing pass,
     summed atomically over grid rows when there are several); otherwise it
     runs one lane per record (3; and case 3, whose support is 35 pixels, one
@@ -475,7 +482,7 @@ def test_device_resident_degridding(hpg, oracle_mod, case):
     assert torch.equal(dev, before)
 
 
-@pytest.mark.parametrize("gridder", [2, 4])
+@pytest.mark.parametrize("gridder", [2, 4, 5])
 @pytest.mark.parametrize("shape", [(4096, 2112, 1, 1), (1024, 1024, 2, 5)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_large_grids_global_bins(hpg, oracle_mod, monkeypatch, gridder, shape, mode):
