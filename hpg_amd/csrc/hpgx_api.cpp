@@ -239,7 +239,7 @@ bool use_halo_global(hpgx_state_t st) {
 bool use_quad(hpgx_state_t st) {
   return st->gridder == 5 && (use_halo(st) || use_halo_global(st))
     && st->mode != HPGX_GRID_FP64 && st->fp32_tile == 2
-    && hpgx::quad_lds_bytes(halo_x(st), halo_y(st), int(st->npol)) <= 65536
+    && hpgx::quad_supported(halo_x(st), halo_y(st), int(st->npol))
     && st->taps_capacity < (size_t(1) << 32);
 }
 

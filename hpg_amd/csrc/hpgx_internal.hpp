@@ -156,9 +156,11 @@ size_t halo_entry_bytes(int npol);
 size_t halo_pixels_per_key(int hx, int hy);
 int halo_tile_limit();
 size_t halo_lds_bytes(int hx, int hy, int nbins);
-// LDS of the quadrant kernel (k_quad) and its bin blocks per pass
+// the quadrant kernel (k_quad): its LDS, bin blocks per pass, and whether it
+// takes halo widths hx, hy (its regions fit LDS, the tile + halo its flush)
 size_t quad_lds_bytes(int hx, int hy, int npol);
 int quad_bins_limit();
+bool quad_supported(int hx, int hy, int npol);
 // acc (FP32 grids): 0 FP64 tile sums, 1 FP32 two-sum via returning atomics,
 // 2 FP32 two-sum of (hi, lo) pairs (compare-and-swap in k_tile4, plain
 // read-modify-write in k_quad).  quad: k_quad (acc 0 or 2) instead of k_tile4

@@ -1861,6 +1861,7 @@ __global__ void k_halo(
 constexpr int quad_block = 4 * wave;
 constexpr int quad_edge = tile / 2;
 constexpr int quad_bins_max = 256;  // bin blocks per pass (LDS segment tables)
+constexpr int quad_flush = 8;       // tile + halo pixels per thread in the flush
 // entries whose first CF taps are loaded ahead of their turn
 template <int N>
 constexpr int quad_depth() { return N == 1 ? 4 : 2; }
@@ -2063,10 +2064,12 @@ __global__ __launch_bounds__(quad_block) void k_quad(
     const float2* slice[N];
     float2 c[N];
   };
-  // The first step's taps load unconditionally, lanes outside the support
-  // reading tap 0 (every slice index points at a whole slice): a load under
-  // a condition would need its register cleared first, which waits for all
-  // loads in flight.
+  // Byte offset of this lane's first-step tap in a slice, for the support
+  // last seen (lanes outside the support read tap 0).  The first step's
+  // taps load unconditionally: a load under a condition would need its
+  // register cleared first, which waits for all loads in flight.
+  int tap_key = -1;
+  unsigned tap_off = 0;
   auto fetch = [&](unsigned i, Slot& s) {
     const QuadRec<N>& p = list[i];
     const unsigned w0v = p.w0;
@@ -2075,16 +2078,20 @@ __global__ __launch_bounds__(quad_block) void k_quad(
     for (int v = 0; v < N; ++v) slv[v] = p.slice[v];
     s.i = i;
     s.w0 = uniform_u(w0v);
-    const int sx = int((s.w0 >> 12) & 63u);
-    const int sy = int((s.w0 >> 18) & 63u);
-    int X, Y;
-    tap_xy(sx, __builtin_amdgcn_rcpf(float(sx)), 0, X, Y);
-    const bool ok = unsigned(X) < unsigned(sx) && unsigned(Y) < unsigned(sy);
-    const int tp = ok ? Y * sx + X : 0;
+    const int key_s = int((s.w0 >> 12) & 4095u);  // sx | sy << 6
+    if (key_s != __builtin_amdgcn_readfirstlane(tap_key)) {
+      tap_key = key_s;
+      const int sx = key_s & 63, sy = key_s >> 6;
+      int X, Y;
+      tap_xy(sx, __builtin_amdgcn_rcpf(float(sx)), 0, X, Y);
+      const bool ok = unsigned(X) < unsigned(sx) && unsigned(Y) < unsigned(sy);
+      tap_off = unsigned(ok ? Y * sx + X : 0) * unsigned(sizeof(float2));
+    }
 #pragma unroll
     for (int v = 0; v < N; ++v) {
       s.slice[v] = taps + uniform_u(slv[v]);
-      s.c[v] = s.slice[v][tp];
+      s.c[v] = *reinterpret_cast<const float2*>(
+        reinterpret_cast<const char*>(s.slice[v]) + tap_off);
     }
   };
   auto add_tap = [&](int px, float h_r, float h_i, float l_r, float l_i) {
@@ -2234,8 +2241,10 @@ __global__ __launch_bounds__(quad_block) void k_quad(
     __syncthreads();
 
     // this wave's entries of the round, D in flight
-    const unsigned beg = (w > 0 ? c0 : 0u) + (w > 1 ? c1 : 0u) + (w > 2 ? c2 : 0u);
-    const unsigned end = beg + (w == 0 ? c0 : w == 1 ? c1 : w == 2 ? c2 : c3);
+    // (wave-uniform: kept in scalar registers, so that the loops below index
+    // the list with scalar arithmetic)
+    const unsigned beg = uniform_u((w > 0 ? c0 : 0u) + (w > 1 ? c1 : 0u) + (w > 2 ? c2 : 0u));
+    const unsigned end = uniform_u(beg + (w == 0 ? c0 : w == 1 ? c1 : w == 2 ? c2 : c3));
     constexpr int D = quad_depth<N>();
     Slot sl[D];
     if constexpr (ONE) {
@@ -2280,7 +2289,17 @@ __global__ __launch_bounds__(quad_block) void k_quad(
   const int R = RX * RY;
   const int hs = halo_pixels(hx, hy);
   double2* hk = halo + size_t(key) * hs;
-  for (int k = threadIdx.x; k < R; k += quad_block) {
+  // every pixel's value first, with the grid loads of the tile's pixels
+  // issued together, then the stores (at most quad_flush pixels per thread:
+  // the quadrant regions fit LDS only for hx, hy <= 13)
+  float4 val[quad_flush];
+  float2 g0[quad_flush];
+#pragma unroll
+  for (int j = 0; j < quad_flush; ++j) {
+    const int k = int(threadIdx.x) + j * quad_block;
+    val[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    g0[j] = make_float2(0.0f, 0.0f);
+    if (k >= R) continue;
     const int ry = k / RX;
     const int rx = k - ry * RX;
     // (hi, lo) of the pixel: the quadrants' pairs combined by two-sum
@@ -2308,18 +2327,29 @@ __global__ __launch_bounds__(quad_block) void k_quad(
       }
       first = false;
     }
+    val[j] = make_float4(fr, fi, flr, fli);
+    const int xx = x0 + rx, yy = y0 + ry;
+    if (rx < tile && ry < tile && xx < geo.nx && yy < geo.ny)
+      g0[j] = static_cast<const float2*>(grid)[(size_t(plane) * geo.ny + yy) * geo.nx + xx];
+  }
+#pragma unroll
+  for (int j = 0; j < quad_flush; ++j) {
+    const int k = int(threadIdx.x) + j * quad_block;
+    if (k >= R) continue;
+    const int ry = k / RX;
+    const int rx = k - ry * RX;
+    const float4 v = val[j];
     if (rx < tile && ry < tile) {
       const int xx = x0 + rx, yy = y0 + ry;
       if (xx >= geo.nx || yy >= geo.ny) continue;
-      float2* gp = static_cast<float2*>(grid) + (size_t(plane) * geo.ny + yy) * geo.nx + xx;
-      const float2 g0 = *gp;
       float sr, er, si, ei;
-      two_sum(g0.x, fr, sr, er);
-      two_sum(g0.y, fi, si, ei);
-      *gp = make_float2(sr + (er + flr), si + (ei + fli));
+      two_sum(g0[j].x, v.x, sr, er);
+      two_sum(g0[j].y, v.y, si, ei);
+      static_cast<float2*>(grid)[(size_t(plane) * geo.ny + yy) * geo.nx + xx] =
+        make_float2(sr + (er + v.z), si + (ei + v.w));
     } else {
       const int h = ry < tile ? ry * hx + (rx - tile) : tile * hx + (ry - tile) * RX + rx;
-      hk[h] = make_double2(double(fr) + double(flr), double(fi) + double(fli));
+      hk[h] = make_double2(double(v.x) + double(v.z), double(v.y) + double(v.w));
     }
   }
   if (threadIdx.x == 0) halo_set[key] = 1u;
@@ -3188,6 +3218,10 @@ size_t quad_lds_bytes(int hx, int hy, int npol) {
     + sizeof(unsigned) * (2 * size_t(quad_bins_max) + 1 + quad_block / wave + 1 + 5);
 }
 int quad_bins_limit() { return quad_bins_max; }
+bool quad_supported(int hx, int hy, int npol) {
+  return hx >= 0 && hy >= 0 && hx < tile && hy < tile && quad_lds_bytes(hx, hy, npol) <= 65536
+    && (tile + hx) * (tile + hy) <= quad_flush * quad_block;
+}
 
 hipError_t launch_tile_halo(
   hipStream_t s, int npol, int mode, int acc, size_t n, size_t nkeys, const Geometry& geo,
@@ -3200,8 +3234,7 @@ hipError_t launch_tile_halo(
   const int nb = global_bins ? 1 : bin_blocks_for(n);
   const unsigned* table = global_bins ? cb.offsets : cb.off_table;
   if (quad) {
-    if (nb > quad_bins_max || quad_lds_bytes(hx, hy, npol) > 65536 || mode != 1
-        || acc != acc_f32_cas)
+    if (nb > quad_bins_max || !quad_supported(hx, hy, npol) || mode != 1 || acc != acc_f32_cas)
       return hipErrorInvalidValue;
     const size_t dyn = 4 * size_t(quad_edge + hx) * (quad_edge + hy) * sizeof(float4);
     // every support within one lane step (hx, hy = largest support - 1)

@@ -203,14 +203,17 @@ def test_fp32_tile_schemes(hpg, oracle_mod, monkeypatch, gridder, scheme, case):
 
 
 @pytest.mark.parametrize("gridder", [4, 5])
-@pytest.mark.parametrize("radius", [(15, 2), (2, 15), (16, 1), (6, 6), (0, 0)])
+@pytest.mark.parametrize("radius", [(15, 2), (2, 15), (16, 1), (6, 6), (12, 1), (15, 0), (15, 1),
+                                    (0, 0)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_support_at_tile_edge(hpg, oracle_mod, monkeypatch, gridder, radius, mode):
     """Largest supports the haloed tiles take (31 pixels on one axis), the
     first they hand to the clipped tiles (33), 13 x 13 (the quadrant kernel's
-    largest regions), and a 1x1 support (no halo) on a grid whose edges are
-    not tile multiples; records cover the grid up to its edges.  No phase
-    gradients: FP64 grids to 1e-12 of max|grid|, FP32-compensated to 1e-6."""
+    largest square regions), wide and flat supports the quadrant kernel takes
+    (25 x 3, 31 x 1) and one whose tile + halo exceeds its flush (31 x 3),
+    and a 1x1 support (no halo) on a grid whose edges are not tile
+    multiples; records cover the grid up to its edges.  No phase gradients:
+    FP64 grids to 1e-12 of max|grid|, FP32-compensated to 1e-6."""
     from hpg_amd import workloads
     monkeypatch.setenv("HPGX_GRIDDER", str(gridder))
     gsz = (97, 75, 1, 1)
