@@ -1879,6 +1879,9 @@ struct QuadRec {
 };
 constexpr unsigned quad_gradient = 1u << 28;
 constexpr unsigned quad_pos_w = 1u << 29;
+constexpr unsigned quad_skip = 1u << 30;  // not added by the pipelined loop
+                                          // (phase gradient, or no used
+                                          // polarization)
 template <int N>
 constexpr int quad_round() { return N == 1 ? quad_block : quad_block / 2; }
 
@@ -2009,6 +2012,9 @@ __global__ __launch_bounds__(quad_block) void k_quad(
 
   for (int k = threadIdx.x; k < 4 * Q; k += quad_block)
     regions[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  // a single CF group is read once (the common case), not once per entry
+  const bool one_group = geo.ngroups == 1;
+  const GroupInfo group0 = groups[0];
   __syncthreads();
 
   const unsigned tid = threadIdx.x;
@@ -2063,6 +2069,7 @@ __global__ __launch_bounds__(quad_block) void k_quad(
     unsigned i, w0;
     const float2* slice[N];
     float2 c[N];
+    unsigned long long lanes;  // ONE: lanes holding a tap of the support
   };
   // Byte offset of this lane's first-step tap in a slice, for the support
   // last seen (lanes outside the support read tap 0).  The first step's
@@ -2070,6 +2077,7 @@ __global__ __launch_bounds__(quad_block) void k_quad(
   // register cleared first, which waits for all loads in flight.
   int tap_key = -1;
   unsigned tap_off = 0;
+  unsigned long long tap_lanes = 0ull;
   auto fetch = [&](unsigned i, Slot& s) {
     const QuadRec<N>& p = list[i];
     const unsigned w0v = p.w0;
@@ -2098,19 +2106,50 @@ __global__ __launch_bounds__(quad_block) void k_quad(
     quad_add<N>(reg32, px, h_r, h_i, l_r, l_i);
   };
   const int lane_px = lane_y8 * QX + lane_x8;
+  // ONE: the raw record of an entry, read from LDS one pipeline stage before
+  // its fetch, so that the fetch's scalar reads find it there
+  struct RawRec {
+    unsigned w0;
+    unsigned sl[N];
+    float2 g[N], gc[N];
+  };
+  auto read_rec = [&](unsigned i, RawRec& r) {
+    const QuadRec<N>& p = list[i];
+    r.w0 = p.w0;
+#pragma unroll
+    for (int v = 0; v < N; ++v) {
+      r.sl[v] = p.slice[v];
+      r.g[v] = p.g[v];
+      r.gc[v] = p.gc[v];
+    }
+  };
+  auto fetch_raw = [&](const RawRec& r, Slot& s) {
+    s.w0 = uniform_u(r.w0);
+    const int key_s = int((s.w0 >> 12) & 4095u);  // sx | sy << 6
+    if (key_s != __builtin_amdgcn_readfirstlane(tap_key)) {
+      tap_key = key_s;
+      const int sx = key_s & 63, sy = key_s >> 6;
+      const bool ok = lane_x8 < sx && lane_y8 < sy;
+      tap_off = unsigned(ok ? lane_y8 * sx + lane_x8 : 0) * unsigned(sizeof(float2));
+      tap_lanes = __ballot(ok);
+    }
+    s.lanes = (s.w0 & quad_skip) ? 0ull : tap_lanes;
+#pragma unroll
+    for (int v = 0; v < N; ++v) {
+      s.slice[v] = taps + uniform_u(r.sl[v]);
+      s.c[v] = *reinterpret_cast<const float2*>(
+        reinterpret_cast<const char*>(s.slice[v]) + tap_off);
+    }
+  };
   // an entry of supports up to 8 x 8 (ONE): a single step, no branch when
   // the entry is live; `live` false (past the wave's entries) adds nothing
-  auto process_one = [&](Slot& s, bool live) {
-    const int sx = int((s.w0 >> 12) & 63u);
-    const int sy = int((s.w0 >> 18) & 63u);
+  auto process_one = [&](Slot& s, const RawRec& r, bool live) {
     const unsigned use = (s.w0 >> 24) & 15u;
-    live = live && !(s.w0 & quad_gradient) && use != 0u;
-    const QuadRec<N>& p = list[s.i];
-    if (live && lane_x8 < sx && lane_y8 < sy) {
+    if (__builtin_amdgcn_inverse_ballot_w64(live ? s.lanes : 0ull)) {
       const int px = int(s.w0 & 4095u) + lane_px;
       if constexpr (N == 1) {
         const float2 c = s.c[0];
-        const float2 g = p.g[0], gc = p.gc[0];
+        const float2 g = r.g[0], gc = r.gc[0];
         const float pr = c.x * g.x - c.y * gc.x;
         const float pi = c.x * g.y + c.y * gc.y;
         add_tap(px, pr, pi, 0.0f, 0.0f);
@@ -2120,7 +2159,7 @@ __global__ __launch_bounds__(quad_block) void k_quad(
         for (int v = 0; v < N; ++v) {
           if (!((use >> v) & 1u)) continue;
           const float2 c = s.c[v];
-          const float2 g = p.g[v], gc = p.gc[v];
+          const float2 g = r.g[v], gc = r.gc[v];
           sr += double(c.x * g.x - c.y * gc.x);
           si += double(c.x * g.y + c.y * gc.y);
         }
@@ -2190,7 +2229,7 @@ __global__ __launch_bounds__(quad_block) void k_quad(
       const bool grad = pos & e4_gradient;
       const int grp = int(e.h.gpk & 0xffffu);
       const int cfcube = int(e.h.gpk >> 16);
-      const GroupInfo g = groups[grp];
+      const GroupInfo g = one_group ? group0 : groups[grp];
       const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
       const long long slice_taps = (long long)g.sx * g.sy;
       const float imf = pos_w ? -1.0f : 1.0f;
@@ -2212,7 +2251,8 @@ __global__ __launch_bounds__(quad_block) void k_quad(
       qd = qx | (qy << 1);
       const unsigned base = unsigned((ly - qy * quad_edge) * QX + (lx - qx * quad_edge));
       rec.w0 = base | (unsigned(g.sx) << 12) | (unsigned(g.sy) << 18) | (use << 24)
-        | (grad ? quad_gradient : 0u) | (pos_w ? quad_pos_w : 0u);
+        | (grad ? quad_gradient : 0u) | (pos_w ? quad_pos_w : 0u)
+        | ((grad || use == 0u) ? quad_skip : 0u);
       if (grad) rec.gc[0].x = __uint_as_float(idx);
     }
     // counting sort by quadrant: one LDS atomic per wave and quadrant
@@ -2237,8 +2277,9 @@ __global__ __launch_bounds__(quad_block) void k_quad(
       const unsigned off = (qd > 0 ? c0 : 0u) + (qd > 1 ? c1 : 0u) + (qd > 2 ? c2 : 0u) + slot;
       list[off] = rec;
     }
-    load_round(r0 + RB);
     __syncthreads();
+    // the next round's entries load while this round's are added
+    load_round(r0 + RB);
 
     // this wave's entries of the round, D in flight
     // (wave-uniform: kept in scalar registers, so that the loops below index
@@ -2248,15 +2289,22 @@ __global__ __launch_bounds__(quad_block) void k_quad(
     constexpr int D = quad_depth<N>();
     Slot sl[D];
     if constexpr (ONE) {
-      // clamped fetches past the end reload the last entry and are not added
+      // records read 2D entries ahead into 2D banks (entry beg + j in bank
+      // j mod 2D, where it stays until it is added), CF taps loaded D
+      // entries ahead; clamped reads past the end repeat the last entry and
+      // are not added
       if (beg < end) {
+        RawRec rr[2 * D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) fetch(min(beg + k, end - 1), sl[k]);
-        for (unsigned i = beg; i < end; i += D) {
+        for (int b = 0; b < 2 * D; ++b) read_rec(min(beg + b, end - 1), rr[b]);
 #pragma unroll
-          for (int k = 0; k < D; ++k) {
-            process_one(sl[k], i + k < end);
-            fetch(min(i + k + D, end - 1), sl[k]);
+        for (int k = 0; k < D; ++k) fetch_raw(rr[k], sl[k]);
+        for (unsigned i = beg; i < end; i += 2 * D) {
+#pragma unroll
+          for (int k = 0; k < 2 * D; ++k) {
+            process_one(sl[k % D], rr[k], i + k < end);
+            fetch_raw(rr[(k + D) % (2 * D)], sl[k % D]);
+            read_rec(min(i + k + 2 * D, end - 1), rr[k]);
           }
         }
       }
