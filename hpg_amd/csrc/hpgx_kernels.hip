@@ -1862,9 +1862,11 @@ constexpr int quad_block = 4 * wave;
 constexpr int quad_edge = tile / 2;
 constexpr int quad_bins_max = 256;  // bin blocks per pass (LDS segment tables)
 constexpr int quad_flush = 8;       // tile + halo pixels per thread in the flush
-// entries whose first CF taps are loaded ahead of their turn
+// entries whose first CF taps are loaded ahead of their turn (2 measured
+// faster than 1, 3 or 4 for one polarization: the pipeline refills at
+// every round)
 template <int N>
-constexpr int quad_depth() { return N == 1 ? 4 : 2; }
+constexpr int quad_depth() { return 2; }
 
 // an entry prepared for its quadrant's wave; gc = (pos_w ? -1 : 1) * g with
 // re and im swapped, so that cmul(conj_w(c), g) = (c.x g.x - c.y gc.x,
@@ -1981,8 +1983,9 @@ __global__ __launch_bounds__(quad_block) void k_quad(
   __shared__ unsigned seg_pref[quad_bins_max + 1];
   __shared__ unsigned seg_beg[quad_bins_max];
   __shared__ unsigned part[quad_block / wave + 1];
-  __shared__ unsigned qcnt[5];  // entries per quadrant, then a gradient flag
-  unsigned* qgrad = qcnt + 4;
+  // per round: entries per quadrant, then a gradient flag; two sets, used
+  // by alternate rounds, so that a round needs no barrier before its counts
+  __shared__ unsigned qcnt2[2][5];
 
   const int QX = quad_edge + hx;
   const int QY = quad_edge + hy;
@@ -2012,6 +2015,7 @@ __global__ __launch_bounds__(quad_block) void k_quad(
 
   for (int k = threadIdx.x; k < 4 * Q; k += quad_block)
     regions[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (threadIdx.x < 10) qcnt2[threadIdx.x / 5][threadIdx.x % 5] = 0u;
   // a single CF group is read once (the common case), not once per entry
   const bool one_group = geo.ngroups == 1;
   const GroupInfo group0 = groups[0];
@@ -2210,13 +2214,14 @@ __global__ __launch_bounds__(quad_block) void k_quad(
       }
     }
   };
-  for (unsigned r0 = 0; r0 < total; r0 += RB) {
+  unsigned round = 0;
+  for (unsigned r0 = 0; r0 < total; r0 += RB, ++round) {
     const unsigned rn = min(unsigned(RB), total - r0);
     const bool mine = tid < rn;
     const Entry4<N> e = ne;
     const unsigned idx = nidx;
-    if (tid < 5) qcnt[tid] = 0u;
-    __syncthreads();  // the previous round's entries are all added
+    unsigned* qcnt = qcnt2[round & 1u];
+    unsigned* qgrad = qcnt + 4;
     QuadRec<N> rec{};
     int qd = -1;
     if (mine) {
@@ -2270,9 +2275,15 @@ __global__ __launch_bounds__(quad_block) void k_quad(
       wbase = __shfl(wbase, 0, wave);
       if (qd == k) slot = wbase + unsigned(__popcll(m & below));
     }
+    // (the list is free: every wave added the previous round's entries
+    // before its counts above)
     __syncthreads();
     const unsigned c0 = qcnt[0], c1 = qcnt[1], c2 = qcnt[2], c3 = qcnt[3];
     const bool any_gradient = *qgrad != 0u;
+    // the other set was last read in the previous round, before its list
+    // barrier; cleared before this round's list barrier, it is ready for
+    // the next round's counts
+    if (tid < 5) qcnt2[(round + 1u) & 1u][tid] = 0u;
     if (mine) {
       const unsigned off = (qd > 0 ? c0 : 0u) + (qd > 1 ? c1 : 0u) + (qd > 2 ? c2 : 0u) + slot;
       list[off] = rec;
