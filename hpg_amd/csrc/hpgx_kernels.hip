@@ -876,6 +876,7 @@ constexpr int bin_records = bin_threads * bin_recs_per_thread;  // per block
 constexpr int bin_keys_max = 8192;
 constexpr int bin_planes_max = 1024;
 constexpr int tile_bins_max = 2048;  // bin blocks per pass (LDS of k_tile3)
+constexpr int bin_stage_bytes = 32768;  // LDS staging of k_bin's haloed entries
 
 // block-wide exclusive scan of a[0, n) in LDS (n <= bin_keys_max), returns
 // the total; `part` holds blockDim.x / wave + 1 values
@@ -1108,16 +1109,24 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
     if (threadIdx.x % wave == 0 && inv) atomicAdd(invalid, inv);
   }
   __syncthreads();
-  block_exclusive_scan(cnt, nkeys, part);
+  const unsigned btotal = block_exclusive_scan(cnt, nkeys, part);
   unsigned* tab = off_table + b * size_t(nkeys + 1);
   for (int k = threadIdx.x; k < nkeys; k += bin_threads) tab[k] = cnt[k];
-  if (threadIdx.x == 0) tab[nkeys] = part[bin_threads / wave];
+  if (threadIdx.x == 0) tab[nkeys] = btotal;
   if (update_weights && local_w)
     for (int k = threadIdx.x; k < nplanes; k += bin_threads)
       if (wl[k] != 0.0) atomicAdd(weights + k, wl[k]);
   __syncthreads();
 
   unsigned* outv = entry_vis + b * block_cap;
+  // HALO: the block-local position of each of this thread's entries (one
+  // per record and grid row with an index; ~0u for none); the entries are
+  // then written through LDS in position order
+  unsigned hpos[bin_recs_per_thread][4];
+#pragma unroll
+  for (int j = 0; j < bin_recs_per_thread; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) hpos[j][r] = ~0u;
 #pragma unroll
   for (int j = 0; j < bin_recs_per_thread; ++j) {
     if (!keep[j]) continue;
@@ -1136,18 +1145,47 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
           const unsigned pos = __hip_atomic_fetch_add(
             cnt + base + ty * geo.ntx + tx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           if constexpr (HALO) {
-            Entry4<N> q;
-            q.h.pos = unsigned(e[j].h.gx - tx0 * tile) | (unsigned(e[j].h.gy - ty0 * tile) << 6)
-              | ((e[j].h.fpk & 0xffffu) << 12) | (pos_w ? e4_pos_w : 0u)
-              | (grad ? e4_gradient : 0u);
-            q.h.gpk = e[j].h.gpk;
-            for (int v = 0; v < N; ++v) q.g[v] = e[j].g[v];
-            static_cast<Entry4<N>*>(entries_out)[b * block_cap + pos] = q;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              if (r == gpol) hpos[j][r] = pos;
           } else {
             static_cast<Entry<N>*>(entries_out)[b * block_cap + pos] = e[j];
           }
           if (grad || degrid_pass) outv[pos] = unsigned(b * bin_records + i);
         }
+    }
+  }
+  if constexpr (HALO) {
+    // chunks of the block's entry range: each thread puts its entries of
+    // the chunk into LDS, then the block copies the chunk out contiguously
+    // (scattered 16 to 40-byte stores would each write a partial line)
+    constexpr unsigned words = unsigned(sizeof(Entry4<N>) / sizeof(unsigned long long));
+    constexpr unsigned chunk = unsigned(bin_stage_bytes / sizeof(Entry4<N>));
+    __shared__ unsigned long long stage[chunk * words];
+    Entry4<N>* staged = reinterpret_cast<Entry4<N>*>(stage);
+    unsigned long long* dst =
+      reinterpret_cast<unsigned long long*>(static_cast<Entry4<N>*>(entries_out) + b * block_cap);
+    for (unsigned c0 = 0; c0 < btotal; c0 += chunk) {
+#pragma unroll
+      for (int j = 0; j < bin_recs_per_thread; ++j) {
+        if (!keep[j]) continue;
+        const bool pos_w = (e[j].h.fpk >> 16) & flag_pos_w;
+        const bool grad = (e[j].h.fpk >> 16) & flag_gradient;
+        Entry4<N> q;
+        q.h.pos = unsigned(e[j].h.gx - tiles_begin(e[j].h.gx) * tile)
+          | (unsigned(e[j].h.gy - tiles_begin(e[j].h.gy) * tile) << 6)
+          | ((e[j].h.fpk & 0xffffu) << 12) | (pos_w ? e4_pos_w : 0u)
+          | (grad ? e4_gradient : 0u);
+        q.h.gpk = e[j].h.gpk;
+        for (int v = 0; v < N; ++v) q.g[v] = e[j].g[v];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (hpos[j][r] - c0 < chunk) staged[hpos[j][r] - c0] = q;
+      }
+      __syncthreads();
+      const unsigned n = min(chunk, btotal - c0) * words;
+      for (unsigned k = threadIdx.x; k < n; k += bin_threads) dst[size_t(c0) * words + k] = stage[k];
+      __syncthreads();
     }
   }
 }
