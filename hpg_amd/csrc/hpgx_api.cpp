@@ -580,6 +580,8 @@ int run_degrid_binned(hpgx_state_t st, size_t n, bool do_grid, const void* recor
   const size_t rec_bytes = visdata_size(st->npol);
   const void* rec_base = records ? records : cb.vis;
   void* out_base = out_records ? out_records : (records ? nullptr : cb.vis);
+  // records returned into another buffer: k_bin writes them there whole
+  const bool copy_out = out_base && out_base != rec_base;
   for (size_t off = 0; off < n; off += chunk) {
     const size_t m = std::min(chunk, n - off);
     CallBuffers c = cb;
@@ -587,7 +589,8 @@ int run_degrid_binned(hpgx_state_t st, size_t n, bool do_grid, const void* recor
     c.gvis = cb.gvis + off * np;
     HPGX_HIP(st, hpgx::launch_bin(
                    st->stream, np, m, geo, st->cf, st->mu, false, false,
-                   hpgx::GvisSource::Values, nullptr, c, block_cap, st->weights, true, true));
+                   hpgx::GvisSource::Values, nullptr, c, block_cap, st->weights, true, true,
+                   copy_out ? static_cast<char*>(out_base) + off * rec_bytes : nullptr));
     HPGX_HIP(st, hpgx::launch_degrid_tiles(
                    st->stream, np, m, st->nkeys, geo, st->cf, st->mu, c, block_cap,
                    halo_x(st), halo_y(st), st->model, cb.degrid_acc + off * np, one_plane,
@@ -1097,10 +1100,11 @@ int hpgx_grid_visibilities(
   }
   if (int rc = ensure_call_buffers(st)) return rc;
   // Device records are read in place unless degridding rewrites them.  The
-  // binned degridder writes the rewritten records straight into
-  // visibilities_out (after a copy of the input when they differ), or
-  // nowhere when they are not returned; other degridders use the staged
-  // copy.
+  // binned degridder reads them in place too and writes the returned
+  // records straight into visibilities_out (its binning pass copies each
+  // record there whole, then the predicted or residual values replace the
+  // record's values), or nowhere when they are not returned; other
+  // degridders use the staged copy.
   const bool dev = memory == HPGX_DEVICE;
   const bool binned_degrid = do_degrid && st->model && use_halo(st);
   const void* records = nullptr;  // nullptr: the staged copy
@@ -1108,15 +1112,8 @@ int hpgx_grid_visibilities(
   if (dev && !do_degrid) {
     records = vis;
   } else if (dev && binned_degrid) {
-    if (return_visibilities) {
-      if (vis_out != vis)
-        HPGX_HIP(st, hipMemcpyAsync(vis_out, vis, n * visdata_size(npol),
-                                    hipMemcpyDeviceToDevice, st->stream));
-      records = vis_out;
-      out_records = vis_out;
-    } else {
-      records = vis;
-    }
+    records = vis;
+    if (return_visibilities) out_records = vis_out;
   } else {
     if (int rc = stage_records(st, vis, n, memory)) return rc;
   }
@@ -1124,7 +1121,7 @@ int hpgx_grid_visibilities(
                         hpgx::GvisSource::Values, nullptr, records, out_records))
     return rc;
   const void* rec_base = records ? records : st->cb.vis;
-  if (return_visibilities && vis_out != rec_base) {
+  if (return_visibilities && !out_records && vis_out != rec_base) {
     HPGX_HIP(st, hipMemcpyAsync(
                    vis_out, rec_base, n * visdata_size(npol),
                    memory == HPGX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,

@@ -150,7 +150,7 @@ hipError_t launch_bin(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool update_weights, bool gvis_ready, GvisSource src,
   const double2* residuals, CallBuffers& cb, size_t block_cap, double* weights,
-  bool halo_entries, bool degrid_pass = false);
+  bool halo_entries, bool degrid_pass = false, void* copy_records = nullptr);
 // haloed tiles (one entry per record and plane; supports below tile)
 size_t halo_entry_bytes(int npol);
 size_t halo_pixels_per_key(int hx, int hy);

@@ -946,7 +946,7 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
   const float2* __restrict__ gvis_in, const double2* __restrict__ residuals,
   unsigned* __restrict__ off_table, void* __restrict__ entries_out,
   unsigned* __restrict__ entry_vis, size_t block_cap, double* __restrict__ weights,
-  unsigned long long* __restrict__ invalid) {
+  unsigned long long* __restrict__ invalid, VisRec<N>* __restrict__ copy_recs) {
 
   __shared__ unsigned cnt[bin_keys_max];
   __shared__ double wl[bin_planes_max];
@@ -1001,6 +1001,15 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
       __builtin_memcpy(&r_copy, wds, sizeof(r_copy));
     }
     const VisRec<N>& r = N <= 2 ? r_copy : recs[i];
+    // degridding that returns the batch: every record is written to the
+    // returned batch here, whose values the degridding kernels then replace
+    if (copy_recs) {
+      constexpr int words = int(sizeof(VisRec<N>) / sizeof(unsigned long long));
+      const unsigned long long* src = reinterpret_cast<const unsigned long long*>(&r);
+      unsigned long long* dst = reinterpret_cast<unsigned long long*>(copy_recs + i);
+#pragma unroll
+      for (int k = 0; k < words; ++k) dst[k] = src[k];
+    }
     const unsigned grp = r.cf_index[1];
     const unsigned cfcube = r.cf_index[0];
     const unsigned gcube = r.grid_cube;
@@ -3239,8 +3248,9 @@ hipError_t launch_bin(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool update_weights, bool gvis_ready, GvisSource src,
   const double2* residuals, CallBuffers& cb, size_t block_cap, double* weights,
-  bool halo_entries, bool degrid_pass) {
+  bool halo_entries, bool degrid_pass, void* copy_records) {
   if (n == 0) return hipSuccess;
+  if (copy_records && !degrid_pass) return hipErrorInvalidValue;
   const int adj = src == GvisSource::Adjoint;
   const int dg = degrid_pass;
   const unsigned nb = unsigned(bin_blocks_for(n));
@@ -3250,7 +3260,7 @@ hipError_t launch_bin(
     static_cast<const VisRec<NP>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,    \
     int(update_weights), int(gvis_ready), adj, dg, cb.gvis, residuals,         \
     cb.off_table, cb.stream_entries, cb.entry_vis, block_cap, weights,         \
-    cb.invalid_count)
+    cb.invalid_count, static_cast<VisRec<NP>*>(copy_records))
   if (halo_entries) {
     switch (npol) {
     case 1: HPGX_BIN(1, true); break;
