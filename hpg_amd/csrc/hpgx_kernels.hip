@@ -2703,8 +2703,18 @@ __global__ __launch_bounds__(block) void k_degrid_lane(
 // chunks whose CF loads issue together.  The per-plane sums go to acc
 // (stored when one plane per record, else added atomically); k_degrid_finish
 // turns them into predictions.
+// model value times the phase screen exp(i ph) (FP64, the reference's
+// cphase).  Not inlined: the FP64 sine and cosine would otherwise raise the
+// register count of k_degrid4's tap loop, which runs them only for records
+// with a phase gradient.
+__device__ __noinline__ double2 degrid_screen(double2 m, double ph) {
+  return cmul(m, cphase(ph));
+}
+// cphase out of line, for the same reason
+__device__ __noinline__ double2 cphase_call(double ph) { return cphase(ph); }
+
 template <int N>
-__global__ __launch_bounds__(halo_block) void k_degrid4(
+__global__ __launch_bounds__(halo_block) __attribute__((amdgpu_waves_per_eu(N == 1 ? 4 : 1))) void k_degrid4(
   Geometry geo, const GroupInfo* __restrict__ groups,
   const float2* __restrict__ taps, Mueller mu, const unsigned* __restrict__ off_table,
   int nbins, size_t block_cap, const Entry4<N>* __restrict__ entries,
@@ -2813,7 +2823,7 @@ __global__ __launch_bounds__(halo_block) void k_degrid4(
       if (grad) {
         const double phX = phi0x + X * dphix;
         const double phY = phi0y + Y * dphiy;
-        m = cmul(m, cphase(-phX - phY));
+        m = degrid_screen(m, -phX - phY);
       }
 #pragma unroll
       for (int v = 0; v < N; ++v) {
@@ -2871,7 +2881,7 @@ __global__ __launch_bounds__(halo_block) void k_degrid4(
     }
     double2 W[N];
     degrid_norms<N>(geo, g, wsum, mu, pos_w, unsigned(cfcube), fxi, fyi, W);
-    const double2 phasor = cphase(recs[i].phase);
+    const double2 phasor = cphase_call(recs[i].phase);
     const double2 conj_phasor = make_double2(phasor.x, -phasor.y);
     double2 pred_d[N];
     for (int v = 0; v < N; ++v) {
