@@ -423,7 +423,7 @@ def main():
                    "device_ms_per_call": degrid_ms,
                    "bytes_per_call": degrid_bytes(NUM_VIS),
                    "hbm_frac": degrid_bytes(NUM_VIS) / (degrid_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                   "kernel": "record copy to the returned batch + k_bin + k_degrid4"},
+                   "kernel": "k_bin (also writes the returned batch) + k_degrid4"},
         "cpu_baseline": None,
     }
     if not args.no_cpu:
