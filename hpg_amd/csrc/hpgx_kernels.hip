@@ -2336,9 +2336,6 @@ __global__ __launch_bounds__(quad_block) void k_quad(
       list[off] = rec;
     }
     __syncthreads();
-    // the next round's entries load while this round's are added
-    load_round(r0 + RB);
-
     // this wave's entries of the round, D in flight
     // (wave-uniform: kept in scalar registers, so that the loops below index
     // the list with scalar arithmetic)
@@ -2357,7 +2354,19 @@ __global__ __launch_bounds__(quad_block) void k_quad(
         for (int b = 0; b < 2 * D; ++b) read_rec(min(beg + b, end - 1), rr[b]);
 #pragma unroll
         for (int k = 0; k < D; ++k) fetch_raw(rr[k], sl[k]);
-        for (unsigned i = beg; i < end; i += 2 * D) {
+        // the next round's entries load while the first taps are in flight
+        load_round(r0 + RB);
+        unsigned i = beg;
+        // every entry added and every read in range: no checks
+        for (; i + 4 * D <= end; i += 2 * D) {
+#pragma unroll
+          for (int k = 0; k < 2 * D; ++k) {
+            process_one(sl[k % D], rr[k], true);
+            fetch_raw(rr[(k + D) % (2 * D)], sl[k % D]);
+            read_rec(i + k + 2 * D, rr[k]);
+          }
+        }
+        for (; i < end; i += 2 * D) {
 #pragma unroll
           for (int k = 0; k < 2 * D; ++k) {
             process_one(sl[k % D], rr[k], i + k < end);
@@ -2365,8 +2374,12 @@ __global__ __launch_bounds__(quad_block) void k_quad(
             read_rec(min(i + k + 2 * D, end - 1), rr[k]);
           }
         }
+      } else {
+        load_round(r0 + RB);
       }
     } else {
+      // the next round's entries load while this round's are added
+      load_round(r0 + RB);
 #pragma unroll
       for (int k = 0; k < D; ++k)
         if (beg + k < end) fetch(beg + k, sl[k]);
