@@ -1861,29 +1861,35 @@ __global__ void k_halo(
   if (x >= geo.nx || y >= geo.ny) return;
   const int hs = halo_pixels(hx, hy);
   const int RX = tile + hx;
-  double sr = 0.0, si = 0.0;
-  bool any = false;
-  if (TX > 0 && u < hx && halo_set[key - 1]) {
-    const double2 h = halo[(key - 1) * hs + v * hx + u];
-    sr += h.x;
-    si += h.y;
-    any = true;
-  }
-  if (TY > 0 && v < hy && halo_set[key - geo.ntx]) {
-    const double2 h = halo[(key - geo.ntx) * hs + tile * hx + v * RX + u];
-    sr += h.x;
-    si += h.y;
-    any = true;
-  }
-  if (TX > 0 && TY > 0 && u < hx && v < hy && halo_set[key - geo.ntx - 1]) {
-    const double2 h = halo[(key - geo.ntx - 1) * hs + tile * hx + v * RX + tile + u];
-    sr += h.x;
-    si += h.y;
-    any = true;
-  }
-  if (!any) return;
+  // the neighbours' flags, their halo values and the grid value load
+  // together (a halo not written by this pass is read but not added)
+  const bool has_l = TX > 0 && u < hx;
+  const bool has_u = TY > 0 && v < hy;
+  const bool has_ul = has_l && has_u;
+  const unsigned set_l = has_l ? halo_set[key - 1] : 0u;
+  const unsigned set_u = has_u ? halo_set[key - geo.ntx] : 0u;
+  const unsigned set_ul = has_ul ? halo_set[key - geo.ntx - 1] : 0u;
+  double2 h_l = make_double2(0.0, 0.0), h_u = h_l, h_ul = h_l;
+  if (has_l) h_l = halo[(key - 1) * hs + v * hx + u];
+  if (has_u) h_u = halo[(key - geo.ntx) * hs + tile * hx + v * RX + u];
+  if (has_ul) h_ul = halo[(key - geo.ntx - 1) * hs + tile * hx + v * RX + tile + u];
   T* gp = grid + (plane * geo.ny + y) * geo.nx + x;
   T g = *gp;
+  if (!(set_l | set_u | set_ul)) return;
+  // added in the order left, upper, upper-left
+  double sr = 0.0, si = 0.0;
+  if (set_l) {
+    sr += h_l.x;
+    si += h_l.y;
+  }
+  if (set_u) {
+    sr += h_u.x;
+    si += h_u.y;
+  }
+  if (set_ul) {
+    sr += h_ul.x;
+    si += h_ul.y;
+  }
   g.x = double(g.x) + sr;
   g.y = double(g.y) + si;
   *gp = g;
