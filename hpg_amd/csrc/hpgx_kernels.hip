@@ -2854,10 +2854,13 @@ __global__ __launch_bounds__(halo_block) __attribute__((amdgpu_waves_per_eu(N ==
         a[v].y += p.y;
       }
     };
-    // any support: taps in slice order, halo_chunk at a time
+    // any support: taps in slice order (t = Y * sx + X), halo_chunk at a
+    // time, the LDS pixel carried from tap to tap
     auto run_chunks = [&]() {
       const int ntap = sx * sy;
-      int cx = 0, cy = 0;
+      int cx = 0, cy = 0;          // (X, Y) of the chunk's first tap
+      int lc = ly * RX + lx;       // its LDS pixel
+      const int wrap = RX - sx;    // pixel step from a row end to the next row
       auto do_chunk = [&](auto full, int t0) {
         constexpr bool FULL = decltype(full)::value;
         bool ok[halo_chunk];
@@ -2869,11 +2872,11 @@ __global__ __launch_bounds__(halo_block) __attribute__((amdgpu_waves_per_eu(N ==
           ok[k] = FULL || t0 + k < ntap;
           Xk[k] = cx;
           Yk[k] = cy;
-          mv[k] = ok[k] ? mreg[(ly + cy) * RX + lx + cx] : make_double2(0.0, 0.0);
-          if (++cx == sx) {
-            cx = 0;
-            ++cy;
-          }
+          mv[k] = ok[k] ? mreg[lc] : make_double2(0.0, 0.0);
+          const bool eol = ++cx == sx;
+          cx = eol ? 0 : cx;
+          cy += eol ? 1 : 0;
+          lc += eol ? 1 + wrap : 1;
 #pragma unroll
           for (int v = 0; v < N; ++v) {
             float2 tv = make_float2(0.0f, 0.0f);
