@@ -385,6 +385,58 @@ def test_shift_cycle(hpg, shape):
     assert np.array_equal(gs.model_values(), m)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("gridder", [3, 5])
+@pytest.mark.parametrize("case", [0, 1])
+def test_zero_model_leaves_gridding_unchanged(hpg, oracle_mod, monkeypatch, case, gridder,
+                                              mode):
+    """ZeroModel (GridderStateUnitTests.cc:1523-1596): degridding against a
+    zero model and gridding the residuals gives the grid and weights of
+    plain gridding (the predictions are zero, so the residuals are the
+    records' values).  Gridder 5 degrids binned, 3 one lane per record;
+    FP64 and FP32-compensated grids, to 1e-12 and 1e-6 of max|grid|."""
+    from hpg_amd import workloads
+    monkeypatch.setenv("HPGX_GRIDDER", str(gridder))
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[case]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=41 + case)
+    rec = workloads.random_visibilities(700, npol, gsz, (1.0, 1.0), cf, seed=42 + case,
+                                        gradient=True, phase=True, margin=-2)
+    print(f"zero model case {case} gridder {gridder} mode {mode}", flush=True)
+    gs_d, _ = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), M, cf, mode, Mc)
+    gs_g, _ = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), M, cf, mode, Mc)
+    gs_d.set_model(hpg.Device.Serial,
+                   np.zeros((gsz[3], gsz[2], gsz[1], gsz[0]), dtype=np.complex128))
+    gs_d.degrid_grid_visibilities(hpg.Device.Serial, rec)
+    gs_g.grid_visibilities(hpg.Device.Serial, rec)
+    a, b = gs_d.grid_values(), gs_g.grid_values()
+    assert np.abs(b).max() > 0
+    tol = 1e-12 if mode == 0 else 1e-6
+    np.testing.assert_allclose(a, b, rtol=0, atol=tol * np.abs(b).max())
+    np.testing.assert_allclose(gs_d.grid_weights(), gs_g.grid_weights(), rtol=1e-13)
+
+
+def test_model_fft_linearity(hpg):
+    """ModelFFT (GridderStateUnitTests.cc:1600-1700): the model FFT divided by
+    its norm is linear in the model, and it is numpy's forward 2-D DFT (the
+    model's default sign is NEGATIVE) of every plane, to 1e-12."""
+    gsz = (64, 48, 2, 2)
+    gs1 = hpg.GridderState.create(hpg.Device.Hip, 0, 15, None, gsz, (0.1, 0.1),
+                                  [[0, -1], [-1, 0]])
+    gs2 = hpg.GridderState.create(hpg.Device.Hip, 0, 15, None, gsz, (0.1, 0.1),
+                                  [[0, -1], [-1, 0]])
+    rng = np.random.default_rng(43)
+    shape = (gsz[3], gsz[2], gsz[1], gsz[0])
+    m = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    gs1.set_model(hpg.Device.Serial, m)
+    gs1.apply_model_fft(norm=1.0)
+    gs2.set_model(hpg.Device.Serial, 2.5 * m)
+    gs2.apply_model_fft(norm=2.5)
+    f1, f2 = gs1.model_values(), gs2.model_values()
+    ref = np.fft.fft2(m, axes=(-2, -1))
+    np.testing.assert_allclose(f1, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    np.testing.assert_allclose(f2, f1, rtol=0, atol=1e-12 * np.abs(f1).max())
+
+
 def test_one_fft_sign(hpg):
     """OneFFT (GridderStateUnitTests.cc:1704-1740)."""
     gs = hpg.GridderState.create(hpg.Device.Hip, 0, 15, None, (2048, 2048, 1, 1),
