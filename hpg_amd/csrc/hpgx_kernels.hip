@@ -2269,6 +2269,7 @@ __global__ __launch_bounds__(quad_block) void k_quad(
   };
   unsigned round = 0;
   for (unsigned r0 = 0; r0 < total; r0 += RB, ++round) {
+    __builtin_amdgcn_s_setprio(0);
     const unsigned rn = min(unsigned(RB), total - r0);
     const bool mine = tid < rn;
     const Entry4<N> e = ne;
@@ -2347,6 +2348,22 @@ __global__ __launch_bounds__(quad_block) void k_quad(
     // the list with scalar arithmetic)
     const unsigned beg = uniform_u((w > 0 ? c0 : 0u) + (w > 1 ? c1 : 0u) + (w > 2 ? c2 : 0u));
     const unsigned end = uniform_u(beg + (w == 0 ? c0 : w == 1 ? c1 : w == 2 ? c2 : c3));
+    // issue priority by the share of the round: the wave with the most
+    // entries sets the round's length, so it goes first on its SIMD (whose
+    // other waves belong to other workgroups)
+    {
+      const unsigned cnt[4] = {c0, c1, c2, c3};
+      const unsigned own = end - beg;
+      int rank = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) rank += (cnt[k] < own || (cnt[k] == own && k < w)) ? 1 : 0;
+      if (rank == 3)
+        __builtin_amdgcn_s_setprio(3);
+      else if (rank == 2)
+        __builtin_amdgcn_s_setprio(2);
+      else if (rank == 1)
+        __builtin_amdgcn_s_setprio(1);
+    }
     constexpr int D = quad_depth<N>();
     Slot sl[D];
     if constexpr (ONE) {
@@ -2406,6 +2423,7 @@ __global__ __launch_bounds__(quad_block) void k_quad(
           quad_gradient_entry<N>(list[i], __float_as_uint(list[i].gc[0].x), geo, taps, entries,
                                  entry_vis, recs, QX, reg32);
   }
+  __builtin_amdgcn_s_setprio(0);
   __syncthreads();
 
   // flush: the sum of the quadrant regions over the tile and its halo
