@@ -2809,6 +2809,18 @@ __global__ __launch_bounds__(halo_block) __attribute__((amdgpu_waves_per_eu(N ==
   const int lane = threadIdx.x % wave;
   constexpr int nw = halo_block / wave;
   for (unsigned base = w * wave; base < total; base += nw * wave) {
+    // issue priority by the passes this wave has left: near the end the
+    // waves of the keys with the most entries, which set the kernel's
+    // length, go first on their SIMD
+    {
+      const unsigned left = (total - base + unsigned(nw * wave) - 1u) / unsigned(nw * wave);
+      if (left >= 3u)
+        __builtin_amdgcn_s_setprio(3);
+      else if (left == 2u)
+        __builtin_amdgcn_s_setprio(2);
+      else
+        __builtin_amdgcn_s_setprio(1);
+    }
     const unsigned q = base + lane;
     if (q >= total) continue;
     int lo = 0, hi = nbins;  // seg_pref[lo] <= q < seg_pref[hi]
