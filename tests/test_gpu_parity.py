@@ -172,6 +172,44 @@ def test_each_gridder_vs_oracle(hpg, oracle_mod, monkeypatch, gridder, case, mod
     np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("gridder", [4, 5])
+def test_dense_tile_vs_oracle(hpg, oracle_mod, monkeypatch, gridder, mode):
+    """4000 single-pol records on one (plane, tile) key: 85% with their
+    support corner in a 6 x 6 box of one quadrant, the rest over the whole
+    tile, a few with phase gradients.  One key then takes many entry rounds
+    with very unequal quadrant shares (gridder 5) or many scheduling rounds
+    (gridder 4).  Against the oracle: e_max, e_rms <= 1e-6 (FP64 mode: the
+    gradient screens' FP32 rounding; FP32-compensated: the pairs)."""
+    from hpg_amd import workloads
+    monkeypatch.setenv("HPGX_GRIDDER", str(gridder))
+    gsz = (96, 96, 1, 1)
+    cf = workloads.random_cf(10, [3], nmueller=1, ncube=1, seed=61)
+    n = 4000
+    rec = workloads.random_visibilities(n, 1, gsz, (1.0, 1.0), cf, seed=62, phase=True,
+                                        gradient=True)
+    keep = rec["cf_phase_gradient"][::50].copy()
+    rec["cf_phase_gradient"] = 0.0
+    rec["cf_phase_gradient"][::50] = keep
+    rng = np.random.default_rng(63)
+    dense = rng.random(n) < 0.85
+    # grid position = u + 48; tile (1, 1) covers 32..63, its first quadrant's
+    # support corners (position - 3) lie in 32..47
+    pos = np.where(dense[:, None], rng.uniform(38.0, 44.0, (n, 2)),
+                   rng.uniform(36.0, 60.0, (n, 2)))
+    rec["uvw"][:, 0] = pos[:, 0] - 48.0
+    rec["uvw"][:, 1] = pos[:, 1] - 48.0
+    print(f"dense tile gridder {gridder} mode {mode}", flush=True)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), [[0]], cf, mode, [[0]])
+    gs.grid_visibilities(hpg.Device.Serial, rec, True)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    got, exp = gs.grid_values(), o.grid_values()
+    assert np.abs(exp).max() > 0
+    e_max, e_rms = rel_errors(got[0, 0], exp[0, 0])
+    assert e_max <= 1e-6 and e_rms <= 1e-6, (e_max, e_rms)
+    np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
+
+
 @pytest.mark.parametrize("gridder", [4, 5])
 @pytest.mark.parametrize("scheme", ["cas", "rtn", "fp64"])
 @pytest.mark.parametrize("case", range(len(CASES)))
