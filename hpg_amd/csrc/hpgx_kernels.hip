@@ -1436,6 +1436,19 @@ __global__ __launch_bounds__(block) void k_tile3(
 // order.  Requires hx, hy < tile.
 
 constexpr int halo_block = 256;
+
+// The dispatcher hands workgroups to the 8 XCDs in turn (workgroup b to
+// XCD b mod 8).  The keyed kernels take key xcd_key(b) instead of b, so
+// that each XCD works through one contiguous run of keys: neighbouring
+// keys' bin segments and offset-table entries share cache lines, which
+// are then fetched into one XCD's L2 instead of several.  A bijection on
+// [0, nkeys).
+constexpr unsigned xcd_count = 8;
+__device__ __forceinline__ unsigned xcd_key(unsigned b, unsigned nkeys) {
+  const unsigned x = b % xcd_count, j = b / xcd_count;
+  const unsigned q = nkeys / xcd_count, r = nkeys % xcd_count;
+  return x * q + min(x, r) + j;
+}
 constexpr int halo_chunk = 8;     // CF taps loaded together per lane
 constexpr int halo_classes = 32;  // LDS bank classes of the entry schedule
 constexpr int halo_sched = 2048;  // entries per scheduling round
@@ -1517,7 +1530,7 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
   unsigned* cls = part + (halo_block / wave + 1);  // bank classes: counts, then cursors
   unsigned* sched = cls + halo_classes;            // entry indexes of one round
 
-  const unsigned key = blockIdx.x;
+  const unsigned key = xcd_key(blockIdx.x, gridDim.x);  // one block per key
   const int nkeys = geo.ntx * geo.nty * geo.nr * geo.nc;
   for (int bb = threadIdx.x; bb < nbins; bb += halo_block) {
     const unsigned* tab = off_table + size_t(bb) * (nkeys + 1);
@@ -2043,7 +2056,7 @@ __global__ __launch_bounds__(quad_block) void k_quad(
   const int QX = quad_edge + hx;
   const int QY = quad_edge + hy;
   const int Q = QX * QY;
-  const unsigned key = blockIdx.x;
+  const unsigned key = xcd_key(blockIdx.x, gridDim.x);  // one block per key
   const int nkeys = geo.ntx * geo.nty * geo.nr * geo.nc;
   for (int bb = threadIdx.x; bb < nbins; bb += quad_block) {
     const unsigned* tab = off_table + size_t(bb) * (nkeys + 1);
@@ -2776,7 +2789,7 @@ __global__ __launch_bounds__(halo_block) __attribute__((amdgpu_waves_per_eu(N ==
   unsigned* seg_beg = seg_pref + nbins + 1;
   unsigned* part = seg_beg + nbins;
 
-  const unsigned key = blockIdx.x;
+  const unsigned key = xcd_key(blockIdx.x, gridDim.x);  // one block per key
   const int nkeys = geo.ntx * geo.nty * geo.nr * geo.nc;
   for (int bb = threadIdx.x; bb < nbins; bb += halo_block) {
     const unsigned* tab = off_table + size_t(bb) * (nkeys + 1);
