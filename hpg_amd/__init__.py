@@ -155,10 +155,20 @@ class CFArray:
 
     def __init__(self, oversampling, groups):
         self._os = int(oversampling)
-        self._groups = [np.ascontiguousarray(g, dtype=np.complex64) for g in groups]
-        for g in self._groups:
+        self._groups = []
+        for g in groups:
+            if _is_device_tensor(g):
+                import torch
+                if g.dtype != torch.complex64 or not g.is_contiguous():
+                    raise ValueError("device CF groups must be contiguous complex64 tensors")
+            else:
+                g = np.ascontiguousarray(g, dtype=np.complex64)
             if g.ndim != 4:
                 raise ValueError("CF group arrays must have shape (cube, mueller, y, x)")
+            self._groups.append(g)
+        kinds = {_is_device_tensor(g) for g in self._groups}
+        if len(kinds) > 1:
+            raise ValueError("CF groups must all be host arrays or all device tensors")
 
     def oversampling(self):
         return self._os
@@ -181,6 +191,18 @@ class CFArray:
                 ((ext[1] - 2 * self.padding * os_) // os_) // 2)
 
 
+def _is_device_tensor(a):
+    return hasattr(a, "data_ptr") and getattr(a, "is_cuda", False)
+
+
+def _order_after_torch(device=None):
+    """Makes work queued on torch's current stream (which may still be
+    writing a device buffer handed to the C ABI) complete before the state's
+    own stream reads it."""
+    import torch
+    torch.cuda.current_stream(device).synchronize()
+
+
 def _check(rc, state):
     if rc != 0:
         raise Error(_lib.error_message(state), rc - 1)
@@ -198,8 +220,7 @@ def _as_records(vis, npol):
                 raise ValueError("device visibilities must be contiguous VisData records")
             # the state's stream reads the records in place: work queued on
             # torch's stream that writes them must be done first
-            import torch
-            torch.cuda.current_stream(vis.device).synchronize()
+            _order_after_torch(vis.device)
             return ctypes.c_void_p(vis.data_ptr()), nbytes // itemsize, 1, vis
     arr = np.asarray(vis)
     dt = visdata_dtype(npol)
@@ -334,16 +355,28 @@ class GridderState:
             *[int(e) for g in range(ng) for e in cf.extents(g)])
         keep = []
         ptrs = (ctypes.c_void_p * max(ng, 1))()
-        memory = 0
+        kinds = {_is_device_tensor(cf.values(g)) for g in range(ng)}
+        if len(kinds) > 1:
+            raise ValueError("CF groups must all be host arrays or all device tensors")
+        memory = 1 if kinds == {True} else 0
         for g in range(ng):
             v = cf.values(g)
-            if hasattr(v, "data_ptr") and getattr(v, "is_cuda", False):
-                memory = 1
+            if memory:
+                import torch
+                if (v.dtype != torch.complex64 or not v.is_contiguous()
+                        or v.numel() != int(np.prod(cf.extents(g)))):
+                    raise ValueError(
+                        f"device CF group {g} must be a contiguous complex64 tensor of "
+                        f"shape (cube, mueller, y, x) = {tuple(cf.extents(g))[::-1]}")
                 ptrs[g] = v.data_ptr()
             else:
                 v = np.ascontiguousarray(v, dtype=np.complex64)
+                if v.size != int(np.prod(cf.extents(g))):
+                    raise ValueError(f"CF group {g} does not match its extents")
                 ptrs[g] = v.ctypes.data
             keep.append(v)
+        if memory:
+            _order_after_torch(keep[0].device)
         _check(_lib.lib().hpgx_set_convolution_function(
             self._h, cf.oversampling(), ng, ext, ptrs, memory), self._h)
         return self
@@ -355,9 +388,14 @@ class GridderState:
         (x, y, mrow, cube) (Layout.Right), or a torch complex128 GPU tensor."""
         if Device(host_device) not in host_devices():
             raise Error("Requested host device is not enabled", ErrorType.DisabledHostDevice)
-        if hasattr(values, "data_ptr") and getattr(values, "is_cuda", False):
-            shape = tuple(values.shape)
-            ptr, memory, keep = ctypes.c_void_p(values.data_ptr()), 1, values
+        if _is_device_tensor(values):
+            import torch
+            keep = values.to(torch.complex128).contiguous()
+            shape = tuple(keep.shape)
+            ptr, memory = ctypes.c_void_p(keep.data_ptr()), 1
+            # the conversion (or the caller's own kernels) must have written
+            # the values before the state's stream copies them
+            _order_after_torch(keep.device)
         else:
             keep = np.ascontiguousarray(values, dtype=np.complex128)
             shape = keep.shape
@@ -452,6 +490,9 @@ class GridderState:
             import torch
             rkeep = residuals.to(torch.complex128).contiguous()
             rptr = ctypes.c_void_p(rkeep.data_ptr())
+            if rkeep.numel() != n * npol:
+                raise ValueError("residuals must hold npol values per record")
+            _order_after_torch(rkeep.device)
         else:
             if memory != 0:
                 raise ValueError("residuals and visibilities must be in the same memory")

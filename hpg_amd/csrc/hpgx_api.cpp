@@ -77,15 +77,13 @@ struct hpgx_state_s {
   // atomics (falls back to 3 when a CF support reaches the tile edge; grids
   // with more (plane, tile) keys than one workgroup's LDS histogram holds
   // take global bins), 3 = block-local bins of one entry per touched tile,
-  // 2 = compact entries binned through global counters, 1 = per-visibility
-  // records gathered through an index
+  // 2 = compact entries binned through global counters
   int gridder = 5;
   size_t nkeys = 0;
   size_t pass_blocks = 0;  // bin blocks per pass of the block-local pipelines
   // tile accumulation of FP32 grids in the haloed-tile gridders: 2 = FP32
   // hi/lo pairs updated by two-sum (default; under compare-and-swap in
-  // gridder 4), 1 = FP32 two-sum through returning FP32 LDS atomics
-  // (gridder 4 only), 0 = FP64 tile sums
+  // gridder 4), 0 = FP64 tile sums
   int fp32_tile = 2;
 
   Plan plan_z2z, plan_c2c;
@@ -272,7 +270,6 @@ int ensure_call_buffers(hpgx_state_t st) {
   if (!cb.vis) {
     const size_t n = std::max<size_t>(st->max_batch, 1);
     HPGX_HIP(st, dev_alloc(cb.vis, n * visdata_size(st->npol)));
-    HPGX_HIP(st, dev_alloc(cb.prep, n * sizeof(hpgx::PrepVis)));
     HPGX_HIP(st, dev_alloc(cb.gvis, n * st->npol * sizeof(float2)));
     HPGX_HIP(st, dev_alloc(cb.invalid_count, sizeof(unsigned long long)));
     HPGX_HIP(st, hipMemsetAsync(cb.invalid_count, 0, sizeof(unsigned long long), st->stream));
@@ -292,10 +289,9 @@ int ensure_call_buffers(hpgx_state_t st) {
     // batches are gridded in chunks (run_grid)
     const size_t epr = entries_per_record(st);
     const bool halo = use_halo(st) || use_halo_global(st);
-    const size_t per_entry = st->gridder == 1
-      ? sizeof(unsigned)
-      : (halo ? hpgx::halo_entry_bytes(int(st->npol)) : hpgx::entry_bytes(int(st->npol)))
-        + sizeof(unsigned);
+    const size_t per_entry =
+      (halo ? hpgx::halo_entry_bytes(int(st->npol)) : hpgx::entry_bytes(int(st->npol)))
+      + sizeof(unsigned);
     const size_t budget = std::max(entry_budget_bytes() / per_entry, epr);
     size_t need = std::min(st->vis_capacity * epr, budget);
     if (use_block_local(st)) {
@@ -332,18 +328,13 @@ int ensure_call_buffers(hpgx_state_t st) {
       }
     }
     if (need > cb.entries_capacity) {
-      dev_free(cb.entries);
       dev_free(cb.stream_entries);
       dev_free(cb.entry_vis);
       cb.entries_capacity = 0;
-      if (st->gridder == 1) {
-        HPGX_HIP(st, dev_alloc(cb.entries, need * sizeof(unsigned)));
-      } else {
-        HPGX_HIP(st, dev_alloc(cb.stream_entries,
-                               need * std::max(hpgx::entry_bytes(int(st->npol)),
-                                               hpgx::halo_entry_bytes(int(st->npol)))));
-        HPGX_HIP(st, dev_alloc(cb.entry_vis, need * sizeof(unsigned)));
-      }
+      HPGX_HIP(st, dev_alloc(cb.stream_entries,
+                             need * std::max(hpgx::entry_bytes(int(st->npol)),
+                                             hpgx::halo_entry_bytes(int(st->npol)))));
+      HPGX_HIP(st, dev_alloc(cb.entry_vis, need * sizeof(unsigned)));
       cb.entries_capacity = need;
     }
   }
@@ -367,12 +358,10 @@ void free_state(hpgx_state_t st) {
   dev_free(st->staging);
   CallBuffers& cb = st->cb;
   dev_free(cb.vis);
-  dev_free(cb.prep);
   dev_free(cb.gvis);
   dev_free(cb.hist);
   dev_free(cb.offsets);
   dev_free(cb.cursor);
-  dev_free(cb.entries);
   dev_free(cb.stream_entries);
   dev_free(cb.entry_vis);
   dev_free(cb.off_table);
@@ -660,34 +649,23 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
     const size_t m = std::min(chunk, n - off);
     CallBuffers c = cb;
     c.vis = static_cast<char*>(rec_base) + off * rec_bytes;
-    c.prep = cb.prep + off;
     c.gvis = cb.gvis + off * np;
     const double2* res = residuals ? residuals + off * np : nullptr;
     HPGX_HIP(st, hipMemsetAsync(c.hist, 0, (st->nkeys + 1) * sizeof(unsigned), st->stream));
-    if (st->gridder != 1) {
-      const bool halo = use_halo_global(st);
-      HPGX_HIP(st, hpgx::launch_count(
-                     st->stream, np, m, geo, st->cf, st->mu, update_weights, true, c,
-                     st->weights, halo));
-      HPGX_HIP(st, hpgx::launch_scan(st->stream, st->nkeys, c));
-      HPGX_HIP(st, hpgx::launch_emit(
-                     st->stream, np, m, geo, st->cf, st->mu, gvis_ready, src, res, c, halo));
-      if (halo)
-        HPGX_HIP(st, hpgx::launch_tile_halo(
-                       st->stream, np, mode, st->fp32_tile, m, st->nkeys, geo, st->cf, st->mu,
-                       c, 0, halo_x(st), halo_y(st), st->grid, true, use_quad(st)));
-      else
-        HPGX_HIP(st, hpgx::launch_tile_stream(
-                       st->stream, np, mode, st->nkeys, geo, st->cf, st->mu, c, st->grid));
-    } else {
-      HPGX_HIP(st, hpgx::launch_prep(
-                     st->stream, np, m, geo, st->cf, st->mu, update_weights, true,
-                     gvis_ready, src, res, c, st->weights));
-      HPGX_HIP(st, hpgx::launch_scan(st->stream, st->nkeys, c));
-      HPGX_HIP(st, hpgx::launch_scatter(st->stream, np, m, geo, st->mu, c, st->cf.groups));
-      HPGX_HIP(st, hpgx::launch_tile_grid(
+    const bool halo = use_halo_global(st);
+    HPGX_HIP(st, hpgx::launch_count(
+                   st->stream, np, m, geo, st->cf, st->mu, update_weights, true, c,
+                   st->weights, halo));
+    HPGX_HIP(st, hpgx::launch_scan(st->stream, st->nkeys, c));
+    HPGX_HIP(st, hpgx::launch_emit(
+                   st->stream, np, m, geo, st->cf, st->mu, gvis_ready, src, res, c, halo));
+    if (halo)
+      HPGX_HIP(st, hpgx::launch_tile_halo(
+                     st->stream, np, mode, st->fp32_tile, m, st->nkeys, geo, st->cf, st->mu,
+                     c, 0, halo_x(st), halo_y(st), st->grid, true, use_quad(st)));
+    else
+      HPGX_HIP(st, hpgx::launch_tile_stream(
                      st->stream, np, mode, st->nkeys, geo, st->cf, st->mu, c, st->grid));
-    }
   }
   return HPGX_SUCCESS;
 }
@@ -788,11 +766,11 @@ int hpgx_create(
   st->mode = grid_mode;
   if (const char* a = std::getenv("HPGX_FP32_TILE")) {
     const std::string v(a);
-    st->fp32_tile = v == "fp64" ? 0 : v == "rtn" ? 1 : 2;
+    st->fp32_tile = v == "fp64" ? 0 : 2;
   }
   if (const char* g = std::getenv("HPGX_GRIDDER")) {
     const int v = std::atoi(g);
-    st->gridder = (v >= 1 && v <= 5) ? v : 5;
+    st->gridder = (v >= 2 && v <= 5) ? v : 5;
   }
   // Mueller index matrices padded to 4x4 with -1
   // (copy_mueller_indexes, src/hpg_runtime.hpp:1590-1614)

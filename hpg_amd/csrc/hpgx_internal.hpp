@@ -54,17 +54,8 @@ struct Geometry {
   int ntx, nty;        // tiles per plane along x and y
 };
 
-// Per-visibility values computed once by the preparation kernel.
-struct PrepVis {
-  int gx, gy;            // grid coordinate of the support's corner
-  short fxi, fyi;        // fine offset + F
-  unsigned short grp;    // CF group
-  unsigned short flags;  // bit 0: pos_w; bit 1: inside grid; bit 2: gradient
-  unsigned cfcube;       // CF cube
-  unsigned gcube;        // grid cube
-};
+// flags of a bin entry
 constexpr unsigned short flag_pos_w = 1;
-constexpr unsigned short flag_inside = 2;
 constexpr unsigned short flag_gradient = 4;
 
 enum class GvisSource { Values, Adjoint };
@@ -72,13 +63,11 @@ enum class GvisSource { Values, Adjoint };
 // Device buffers for one gridding call (owned by the state).
 struct CallBuffers {
   void* vis;            // raw VisData<N> records
-  PrepVis* prep;        // per visibility
   float2* gvis;         // per visibility x N
   unsigned* hist;       // per (plane, tile) key, nkeys + 1
   unsigned* offsets;    // exclusive scan of hist, nkeys + 1
   unsigned* cursor;     // scatter positions, nkeys
-  unsigned* entries;    // (vis << 2) | gpol, sorted by key
-  size_t entries_capacity;
+  size_t entries_capacity;  // entries of stream_entries
   void* scan_temp;
   size_t scan_temp_bytes;
   void* stream_entries;  // compact binned entries (Entry<N>), sorted by key
@@ -113,19 +102,9 @@ hipError_t launch_cf_wsum(
 
 size_t tiles_per_footprint(int max_sx, int max_sy);
 
-// prepares visibilities and counts (plane, tile) entries
-hipError_t launch_prep(
-  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
-  const Mueller& mu, bool update_weights, bool do_grid, bool gvis_ready,
-  GvisSource src, const double2* residuals, CallBuffers& cb, double* weights);
+// exclusive scan of the global bin counts (hist -> offsets, cursor)
 hipError_t launch_scan(hipStream_t s, size_t nkeys, CallBuffers& cb);
 size_t scan_temp_bytes(size_t nkeys);
-hipError_t launch_scatter(
-  hipStream_t s, int npol, size_t n, const Geometry& geo, const Mueller& mu,
-  CallBuffers& cb, const GroupInfo* groups);
-hipError_t launch_tile_grid(
-  hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
-  const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid);
 // global bins; halo_entries: one Entry4 per record and plane in the bin of
 // the tile holding the support corner (for launch_tile_halo with
 // global_bins), else one Entry per touched tile

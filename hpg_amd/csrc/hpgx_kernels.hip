@@ -5,23 +5,25 @@
 // that the FP64-grid mode reproduces the reference's per-tap values bit for
 // bit and differs from it only by the order of FP64 accumulation.
 //
-// Gridding pipeline for one batch of visibilities (VisibilityGridder::kernel,
-// src/hpg_core.hpp:1082-1253, re-organized for the device):
-//   k_prep     one thread per visibility: coordinates, skip rule, pre-phased
-//              values (gvis), FP64 weight sums, and a count of the
-//              (plane, tile) bins its support touches;
-//   scan       exclusive sum of the bin counts (rocPRIM via hipCUB);
-//   k_scatter  one thread per visibility: writes (visibility, gpol) entries
-//              into their bins;
-//   k_tile     one workgroup per bin: each wave accumulates its share of the
-//              bin's entries into a private LDS copy of the 32x32 tile (FP64,
-//              or FP32 with two-sum compensation), then the workgroup folds
-//              the copies and adds the tile into the grid with one coalesced
-//              read-modify-write.  Tiles partition each plane, so no global
-//              atomics touch the grid.
-// Degridding (degrid_vis, src/hpg_core.hpp:579-682) is one wave per
-// visibility: lanes gather model pixels and CF taps across the support and
-// the wave reduces with cross-lane shuffles.
+// Gridding (VisibilityGridder::kernel, src/hpg_core.hpp:1082-1253) is
+// re-organized for the device in two stages: binning, then one workgroup per
+// (grid plane, 32 x 32 tile) key accumulating the tile in LDS and adding it
+// into the grid with one coalesced read-modify-write, so that no global
+// atomic touches the grid.
+//   k_bin + k_quad / k_tile4 + k_halo   records binned by the tile holding
+//        their support corner (block-local bins, no global atomics); the
+//        tile kernels accumulate the tile plus a halo of (support - 1)
+//        pixels, and k_halo folds each tile's halo into its neighbours.
+//        Supports narrower than a tile on grids of up to 8192 keys.
+//   k_bin<N,false> + k_tile3            one entry per touched tile, clipped
+//        footprints (supports reaching the tile edge, small grids).
+//   k_count + scan + k_emit + k_tile2   the same entries binned through
+//        global counters (grids with more keys than one workgroup's LDS
+//        histogram holds).
+// Degridding (degrid_vis, src/hpg_core.hpp:579-682): k_degrid4 reads the
+// model's tile and halo from LDS for binned records; k_degrid_lane (one lane
+// per record) and k_degrid (one wave per record, cross-lane reduction)
+// gather the model directly.
 
 #include "hpgx_internal.hpp"
 
@@ -221,305 +223,48 @@ __global__ void k_cf_wsum(
 
 // ---- gridding ----------------------------------------------------------
 
-template <int N>
-__global__ __launch_bounds__(block) void k_prep(
-  const VisRec<N>* __restrict__ recs, size_t n, Geometry geo,
-  const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
-  Mueller mu, int update_weights, int do_grid, int gvis_ready, int adjoint,
-  const double2* __restrict__ residuals, PrepVis* __restrict__ prep,
-  float2* __restrict__ gvis, unsigned* __restrict__ hist,
-  double* __restrict__ weights, unsigned long long* __restrict__ invalid) {
-
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  bool counted = false;  // contributes to the weights
-  unsigned gcube = 0;
-  double tw[4] = {0.0, 0.0, 0.0, 0.0};
-
-  if (i < n) {
-    const VisRec<N>& r = recs[i];
-    const unsigned grp = r.cf_index[1];
-    const unsigned cfcube = r.cf_index[0];
-    gcube = r.grid_cube;
-    PrepVis p;
-    p.gx = p.gy = 0;
-    p.fxi = p.fyi = 0;
-    p.grp = 0;
-    p.flags = 0;
-    p.cfcube = cfcube;
-    p.gcube = gcube;
-    bool valid = grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc);
-    if (valid) valid = cfcube < unsigned(groups[grp].ncube);
-    if (!valid) {
-      atomicAdd(invalid, 1ull);
-    } else {
-      const GroupInfo g = groups[grp];
-      const Coord c = vis_coord(r, geo, g);
-      const bool pos_w = r.uvw[2] > 0;
-      p.gx = c.gx;
-      p.gy = c.gy;
-      p.fxi = short(c.fx + geo.F);
-      p.fyi = short(c.fy + geo.F);
-      p.grp = (unsigned short)grp;
-      p.flags = (pos_w ? flag_pos_w : 0) | (c.inside ? flag_inside : 0)
-        | ((r.grad[0] != 0.0 || r.grad[1] != 0.0) ? flag_gradient : 0);
-      const double2 phasor = cphase(r.phase);
-      if (!gvis_ready) {
-        if (!adjoint) {
-          for (int v = 0; v < N; ++v)
-            gvis[i * N + v] = prephase(r.vis[v], phasor, r.w[v]);
-        } else if (c.inside) {
-          double2 W[N];
-          degrid_norms<N>(geo, g, wsum, mu, pos_w, cfcube, p.fxi, p.fyi, W);
-          for (int v = 0; v < N; ++v) {
-            double2 d = W[v];
-            if (d.x == 0.0 && d.y == 0.0) d = make_double2(1.0, 0.0);
-            d.y = -d.y;
-            gvis[i * N + v] = narrow(kdiv(cmul(residuals[i * N + v], phasor), d));
-          }
-        }
-      }
-      if (c.inside && update_weights) {
-        // twgt = sum_v |sum_XY cf'| * w_v (src/hpg_core.hpp:877-885)
-        counted = true;
-        for (int gpol = 0; gpol < geo.nr; ++gpol) {
-          const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
-          double t = 0.0;
-          for (int v = 0; v < N; ++v) {
-            const int m = row[v];
-            double mag = 0.0;
-            if (m >= 0 && m < g.nmueller) {
-              const double2 s = wsum[g.wsum_offset
-                                     + ((long long)(cfcube * g.nmueller + m) * geo.nf
-                                        + p.fxi) * geo.nf + p.fyi];
-              mag = hypot(s.x, s.y);
-            }
-            t += mag * double(r.w[v]);
-          }
-          tw[gpol] = t;
-        }
-      }
-      if (c.inside && do_grid) {
-        const int ntiles = geo.ntx * geo.nty;
-        const int tx0 = tiles_begin(c.gx), tx1 = tiles_end(c.gx, g.sx);
-        const int ty0 = tiles_begin(c.gy), ty1 = tiles_end(c.gy, g.sy);
-        for (int gpol = 0; gpol < geo.nr; ++gpol) {
-          if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
-          const size_t base = size_t(gcube * geo.nr + gpol) * ntiles;
-          for (int ty = ty0; ty < ty1; ++ty)
-            for (int tx = tx0; tx < tx1; ++tx)
-              atomicAdd(hist + base + ty * geo.ntx + tx, 1u);
-        }
-      }
-    }
-    prep[i] = p;
-  }
-
-  // weights: one FP64 atomic per (wave, grid cube, gpol)
-  if (update_weights) {
-    unsigned long long active = __ballot(counted);
-    const int lane = threadIdx.x % wave;
-    while (active) {
-      const int leader = __ffsll((long long)active) - 1;
-      const unsigned cube = __shfl(gcube, leader, wave);
-      const bool mine = counted && gcube == cube;
-      active &= ~__ballot(mine);
-      for (int gpol = 0; gpol < geo.nr; ++gpol) {
-        const double s = wave_sum(mine ? tw[gpol] : 0.0);
-        if (lane == leader) atomicAdd(weights + size_t(cube) * geo.nr + gpol, s);
-      }
-    }
-  }
-}
-
-template <int N>
-__global__ __launch_bounds__(block) void k_scatter(
-  size_t n, Geometry geo, const GroupInfo* __restrict__ groups, Mueller mu,
-  const PrepVis* __restrict__ prep, unsigned* __restrict__ cursor,
-  unsigned* __restrict__ entries) {
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const PrepVis p = prep[i];
-  if (!(p.flags & flag_inside)) return;
-  const GroupInfo g = groups[p.grp];
-  const bool pos_w = p.flags & flag_pos_w;
-  const int ntiles = geo.ntx * geo.nty;
-  const int tx0 = tiles_begin(p.gx), tx1 = tiles_end(p.gx, g.sx);
-  const int ty0 = tiles_begin(p.gy), ty1 = tiles_end(p.gy, g.sy);
-  for (int gpol = 0; gpol < geo.nr; ++gpol) {
-    if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
-    const size_t base = size_t(p.gcube * geo.nr + gpol) * ntiles;
-    for (int ty = ty0; ty < ty1; ++ty)
-      for (int tx = tx0; tx < tx1; ++tx) {
-        const unsigned pos = atomicAdd(cursor + base + ty * geo.ntx + tx, 1u);
-        entries[pos] = (unsigned(i) << 2) | unsigned(gpol);
-      }
-  }
-}
-
+// s + e = a + b exactly, s = fl(a + b) (Knuth's two-sum)
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   s = a + b;
   const float bb = s - a;
   e = (a - (s - bb)) + (b - bb);
 }
 
-// mode 0: complex<double> grid, FP64 accumulation of FP32 products
-// mode 1: complex<float> grid, FP32 products, two-sum compensated tiles
-template <int N, int MODE>
-__global__ __launch_bounds__(block) void k_tile(
-  Geometry geo, const GroupInfo* __restrict__ groups,
-  const float2* __restrict__ taps, Mueller mu,
-  const unsigned* __restrict__ offsets, const unsigned* __restrict__ entries,
-  const PrepVis* __restrict__ prep, const float2* __restrict__ gvis,
-  const VisRec<N>* __restrict__ recs, void* __restrict__ grid) {
+// An FP32-compensated tile pixel component is a pair (hi, lo) of floats
+// packed into 64 bits (hi in the low word), updated as one unit by
+// compare-and-swap where several lanes may add to it.
+__device__ __forceinline__ unsigned long long pack_pair(float hi, float lo) {
+  return (unsigned long long)__float_as_uint(hi)
+    | ((unsigned long long)__float_as_uint(lo) << 32);
+}
 
-  using Acc = typename std::conditional<MODE == 0, double2, float4>::type;
-  __shared__ Acc acc[tile_waves * tile * tile];
+// the pair (hi, lo) after adding h (+ the low-order part l, when HAS_L)
+// to it: hi' = fl(hi + h), lo' = fl(lo + (err + l)) with err the exact
+// rounding error of the first sum (two-sum)
+template <bool HAS_L>
+__device__ __forceinline__ unsigned long long pair_plus(
+  unsigned long long pair, float h, float l) {
+  const float hi = __uint_as_float(unsigned(pair));
+  const float lo = __uint_as_float(unsigned(pair >> 32));
+  float sum, err;
+  two_sum(hi, h, sum, err);
+  return pack_pair(sum, HAS_L ? lo + (err + l) : lo + err);
+}
 
-  const unsigned key = blockIdx.x;
-  const unsigned beg = offsets[key];
-  const unsigned end = offsets[key + 1];
-  if (beg == end) return;
+// compare-and-swap of the pair at a from `expect` to `desired`; on failure
+// `expect` receives the value found
+__device__ __forceinline__ bool pair_swap(
+  unsigned long long* a, unsigned long long& expect, unsigned long long desired) {
+  return __hip_atomic_compare_exchange_strong(
+    a, &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
-  const int ntiles = geo.ntx * geo.nty;
-  const unsigned plane = key / ntiles;
-  const int t = int(key % ntiles);
-  const int x0 = (t % geo.ntx) * tile;
-  const int y0 = (t / geo.ntx) * tile;
-  const int gpol = int(plane % geo.nr);
-
-  for (int k = threadIdx.x; k < tile_waves * tile * tile; k += block) {
-    if constexpr (MODE == 0)
-      acc[k] = make_double2(0.0, 0.0);
-    else
-      acc[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  }
-  __syncthreads();
-
-  const int w = threadIdx.x / wave;
-  const int lane = threadIdx.x % wave;
-  Acc* my = acc + w * tile * tile;
-
-  for (unsigned e = beg + w; e < end; e += tile_waves) {
-    const unsigned ent = uniform(int(entries[e]));
-    const unsigned i = ent >> 2;
-    const PrepVis p = prep[i];
-    const GroupInfo g = groups[p.grp];
-    const bool pos_w = p.flags & flag_pos_w;
-    const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
-    const float imf = pos_w ? -1.0f : 1.0f;
-
-    const int xa = max(p.gx, x0) - p.gx;
-    const int xb = min(p.gx + g.sx, x0 + tile) - p.gx;
-    const int ya = max(p.gy, y0) - p.gy;
-    const int yb = min(p.gy + g.sy, y0 + tile) - p.gy;
-    const int wdt = xb - xa;
-    const int ntap = wdt * (yb - ya);
-    const unsigned mg = magic(unsigned(wdt));
-
-    const float2* slice[N];
-    float2 gv[N];
-    bool use[N];
-    const long long slice_taps = (long long)g.sx * g.sy;
-    for (int v = 0; v < N; ++v) {
-      const int m = row[v];
-      use[v] = m >= 0 && m < g.nmueller;
-      slice[v] = taps + g.tap_offset
-        + (((long long)(p.cfcube * g.nmueller + (use[v] ? m : 0)) * geo.nf + p.fxi)
-             * geo.nf + p.fyi) * slice_taps;
-      gv[v] = gvis[size_t(i) * N + v];
-    }
-    const bool grad = p.flags & flag_gradient;
-    double phi0x = 0.0, phi0y = 0.0, dphix = 0.0, dphiy = 0.0;
-    if (grad) {
-      // Vis<N> phase screen origin and increment (src/hpg_core.hpp:543-547)
-      const double gxr = recs[i].grad[0];
-      const double gyr = recs[i].grad[1];
-      phi0x = -gxr * ((g.sx / 2) * geo.os - (p.fxi - geo.F));
-      phi0y = -gyr * ((g.sy / 2) * geo.os - (p.fyi - geo.F));
-      dphix = gxr * geo.os;
-      dphiy = gyr * geo.os;
-    }
-
-    for (int q = lane; q < ntap; q += wave) {
-      const int yl = int(div_magic(unsigned(q), unsigned(wdt), mg));
-      const int X = xa + (q - yl * wdt);
-      const int Y = ya + yl;
-      float2 scr = make_float2(1.0f, 0.0f);
-      if (grad) {
-        const double phX = phi0x + X * dphix;
-        const double phY = phi0y + Y * dphiy;
-        scr = narrow(cphase(phX + phY));
-      }
-      const int li = (p.gy + Y - y0) * tile + (p.gx + X - x0);
-      if constexpr (MODE == 0) {
-        double2 s = make_double2(0.0, 0.0);
-        for (int v = 0; v < N; ++v) {
-          if (!use[v]) continue;
-          float2 c = slice[v][Y * g.sx + X];
-          c.y *= imf;
-          if (grad) c = cmul(c, scr);
-          const float2 pr = cmul(c, gv[v]);
-          s.x += double(pr.x);
-          s.y += double(pr.y);
-        }
-        double2 a = my[li];
-        a.x += s.x;
-        a.y += s.y;
-        my[li] = a;
-      } else {
-        float4 a = my[li];
-        for (int v = 0; v < N; ++v) {
-          if (!use[v]) continue;
-          float2 c = slice[v][Y * g.sx + X];
-          c.y *= imf;
-          if (grad) c = cmul(c, scr);
-          const float2 pr = cmul(c, gv[v]);
-          float s, err;
-          two_sum(a.x, pr.x, s, err);
-          a.x = s;
-          a.z += err;
-          two_sum(a.y, pr.y, s, err);
-          a.y = s;
-          a.w += err;
-        }
-        my[li] = a;
-      }
-    }
-  }
-  __syncthreads();
-
-  for (int k = threadIdx.x; k < tile * tile; k += block) {
-    const int xx = x0 + k % tile;
-    const int yy = y0 + k / tile;
-    if (xx >= geo.nx || yy >= geo.ny) continue;
-    const size_t gi = (size_t(plane) * geo.ny + yy) * geo.nx + xx;
-    if constexpr (MODE == 0) {
-      double2 s = acc[k];
-      for (int ww = 1; ww < tile_waves; ++ww) {
-        s.x += acc[ww * tile * tile + k].x;
-        s.y += acc[ww * tile * tile + k].y;
-      }
-      double2* gp = static_cast<double2*>(grid) + gi;
-      double2 gvl = *gp;
-      gvl.x += s.x;
-      gvl.y += s.y;
-      *gp = gvl;
-    } else {
-      float2* gp = static_cast<float2*>(grid) + gi;
-      const float2 g0 = *gp;
-      float sr = g0.x, si = g0.y, cr = 0.0f, ci = 0.0f;
-      for (int ww = 0; ww < tile_waves; ++ww) {
-        const float4 a = acc[ww * tile * tile + k];
-        float s, err;
-        two_sum(sr, a.x, s, err);
-        sr = s;
-        cr += err + a.z;
-        two_sum(si, a.y, s, err);
-        si = s;
-        ci += err + a.w;
-      }
-      *gp = make_float2(sr + cr, si + ci);
-    }
+// retries until the update of the pair at a lands; every retry follows a
+// successful update of the same pair by another lane or wave
+template <bool HAS_L>
+__device__ __forceinline__ void pair_add(
+  unsigned long long* a, unsigned long long expect, float h, float l) {
+  while (!pair_swap(a, expect, pair_plus<HAS_L>(expect, h, l))) {
   }
 }
 
@@ -532,9 +277,9 @@ __global__ __launch_bounds__(block) void k_tile(
 //           contiguous stream instead of gathering per-visibility records
 // k_tile2   one workgroup per bin, one shared LDS tile: waves load 64 entries
 //           at a time (one per lane), broadcast each to the whole wave with
-//           readlane, and accumulate with LDS atomics (FP64 adds, or FP32
-//           adds whose rounding error is recovered from the returned old
-//           value and accumulated in a second array -- compensated two-sum)
+//           readlane, and accumulate with LDS atomics (FP64 adds), or
+//           (FP32-compensated grids) with compare-and-swap updates of
+//           (hi, lo) pairs by two-sum
 
 struct EntryHead {
   int gx, gy;
@@ -716,7 +461,8 @@ __global__ __launch_bounds__(block) void k_tile2(
   constexpr int TT = tile * tile;
   // MODE 0: re[TT], im[TT] (double); MODE 1: hi_re, hi_im, lo_re, lo_im (float)
   __shared__ double acc64[MODE == 0 ? 2 * TT : 1];
-  __shared__ float acc32[MODE == 1 ? 4 * TT : 1];
+  // MODE 1: (hi, lo) pairs of re[TT], then im[TT]
+  __shared__ unsigned long long pairs[MODE == 1 ? 2 * TT : 1];
 
   const unsigned key = blockIdx.x;
   const unsigned beg = offsets[key];
@@ -733,7 +479,7 @@ __global__ __launch_bounds__(block) void k_tile2(
   if constexpr (MODE == 0) {
     for (int k = threadIdx.x; k < 2 * TT; k += block) acc64[k] = 0.0;
   } else {
-    for (int k = threadIdx.x; k < 4 * TT; k += block) acc32[k] = 0.0f;
+    for (int k = threadIdx.x; k < 2 * TT; k += block) pairs[k] = 0ull;
   }
   __syncthreads();
 
@@ -820,13 +566,8 @@ __global__ __launch_bounds__(block) void k_tile2(
           // split the FP64 tap sum into float hi + lo (exact for N == 1)
           const float hr = float(sr), hi = float(si);
           const float lr = float(sr - double(hr)), lim = float(si - double(hi));
-          const float orr = __hip_atomic_fetch_add(acc32 + li, hr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          const float ori = __hip_atomic_fetch_add(acc32 + TT + li, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          float s, er, ei;
-          two_sum(orr, hr, s, er);
-          two_sum(ori, hi, s, ei);
-          __hip_atomic_fetch_add(acc32 + 2 * TT + li, er + lr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_fetch_add(acc32 + 3 * TT + li, ei + lim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          pair_add<(N > 1)>(pairs + li, pairs[li], hr, lr);
+          pair_add<(N > 1)>(pairs + TT + li, pairs[TT + li], hi, lim);
         }
       }
     }
@@ -848,9 +589,11 @@ __global__ __launch_bounds__(block) void k_tile2(
       float2* gp = static_cast<float2*>(grid) + gi;
       const float2 g0 = *gp;
       float sr, er, si, ei;
-      two_sum(g0.x, acc32[k], sr, er);
-      two_sum(g0.y, acc32[TT + k], si, ei);
-      *gp = make_float2(sr + (er + acc32[2 * TT + k]), si + (ei + acc32[3 * TT + k]));
+      const unsigned long long pr = pairs[k], pi = pairs[TT + k];
+      two_sum(g0.x, __uint_as_float(unsigned(pr)), sr, er);
+      two_sum(g0.y, __uint_as_float(unsigned(pi)), si, ei);
+      *gp = make_float2(sr + (er + __uint_as_float(unsigned(pr >> 32))),
+                        si + (ei + __uint_as_float(unsigned(pi >> 32))));
     }
   }
 }
@@ -1215,7 +958,8 @@ __global__ __launch_bounds__(block) void k_tile3(
 
   constexpr int TT = tile * tile;
   __shared__ double acc64[MODE == 0 ? 2 * TT : 1];
-  __shared__ float acc32[MODE == 1 ? 4 * TT : 1];
+  // MODE 1: (hi, lo) pairs of re[TT], then im[TT]
+  __shared__ unsigned long long pairs[MODE == 1 ? 2 * TT : 1];
   __shared__ unsigned seg_pref[tile_bins_max + 1];
   __shared__ unsigned seg_beg[tile_bins_max];
   __shared__ unsigned part[block / wave + 1];
@@ -1244,7 +988,7 @@ __global__ __launch_bounds__(block) void k_tile3(
   if constexpr (MODE == 0) {
     for (int k = threadIdx.x; k < 2 * TT; k += block) acc64[k] = 0.0;
   } else {
-    for (int k = threadIdx.x; k < 4 * TT; k += block) acc32[k] = 0.0f;
+    for (int k = threadIdx.x; k < 2 * TT; k += block) pairs[k] = 0ull;
   }
   __syncthreads();
 
@@ -1258,13 +1002,8 @@ __global__ __launch_bounds__(block) void k_tile3(
     } else {
       const float hr = float(sr), hi = float(si);
       const float lr = float(sr - double(hr)), lim = float(si - double(hi));
-      const float orr = __hip_atomic_fetch_add(acc32 + li, hr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      const float ori = __hip_atomic_fetch_add(acc32 + TT + li, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      float s, er, ei;
-      two_sum(orr, hr, s, er);
-      two_sum(ori, hi, s, ei);
-      __hip_atomic_fetch_add(acc32 + 2 * TT + li, er + lr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      __hip_atomic_fetch_add(acc32 + 3 * TT + li, ei + lim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      pair_add<(N > 1)>(pairs + li, pairs[li], hr, lr);
+      pair_add<(N > 1)>(pairs + TT + li, pairs[TT + li], hi, lim);
     }
   };
 
@@ -1414,9 +1153,11 @@ __global__ __launch_bounds__(block) void k_tile3(
       float2* gp = static_cast<float2*>(grid) + gi;
       const float2 g0 = *gp;
       float sr, er, si, ei;
-      two_sum(g0.x, acc32[k], sr, er);
-      two_sum(g0.y, acc32[TT + k], si, ei);
-      *gp = make_float2(sr + (er + acc32[2 * TT + k]), si + (ei + acc32[3 * TT + k]));
+      const unsigned long long pr = pairs[k], pi = pairs[TT + k];
+      two_sum(g0.x, __uint_as_float(unsigned(pr)), sr, er);
+      two_sum(g0.y, __uint_as_float(unsigned(pi)), si, ei);
+      *gp = make_float2(sr + (er + __uint_as_float(unsigned(pr >> 32))),
+                        si + (ei + __uint_as_float(unsigned(pi >> 32))));
     }
   }
 }
@@ -1461,50 +1202,12 @@ __host__ __device__ inline int halo_pixels(int hx, int hy) {
 
 // tile accumulation schemes of k_tile4
 //   acc_f64       FP64 LDS atomics (FP64 grids; also selectable for FP32 grids)
-//   acc_f32_rtn   FP32 hi/lo per pixel: hi via FP32 LDS atomics returning the
-//                 old value, its exact rounding error (two-sum) into lo
 //   acc_f32_cas   FP32 hi/lo per pixel packed into 64 bits and updated by
 //                 compare-and-swap: hi' = fl(hi + p), lo' = fl(lo + err) with
 //                 err the exact two-sum error, so each update is one
 //                 error-free transformation on one coherent pair
 constexpr int acc_f64 = 0;
-constexpr int acc_f32_rtn = 1;
 constexpr int acc_f32_cas = 2;
-
-__device__ __forceinline__ unsigned long long pack_pair(float hi, float lo) {
-  return (unsigned long long)__float_as_uint(hi)
-    | ((unsigned long long)__float_as_uint(lo) << 32);
-}
-
-// the pair (hi, lo) after adding h (+ the low-order part l, when HAS_L)
-// to it: hi' = fl(hi + h), lo' = fl(lo + (err + l)) with err the exact
-// rounding error of the first sum (two-sum)
-template <bool HAS_L>
-__device__ __forceinline__ unsigned long long pair_plus(
-  unsigned long long pair, float h, float l) {
-  const float hi = __uint_as_float(unsigned(pair));
-  const float lo = __uint_as_float(unsigned(pair >> 32));
-  float sum, err;
-  two_sum(hi, h, sum, err);
-  return pack_pair(sum, HAS_L ? lo + (err + l) : lo + err);
-}
-
-// compare-and-swap of the pair at a from `expect` to `desired`; on failure
-// `expect` receives the value found
-__device__ __forceinline__ bool pair_swap(
-  unsigned long long* a, unsigned long long& expect, unsigned long long desired) {
-  return __hip_atomic_compare_exchange_strong(
-    a, &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// retries until the update of the pair at a lands; every retry follows a
-// successful update of the same pair by another lane or wave
-template <bool HAS_L>
-__device__ __forceinline__ void pair_add(
-  unsigned long long* a, unsigned long long expect, float h, float l) {
-  while (!pair_swap(a, expect, pair_plus<HAS_L>(expect, h, l))) {
-  }
-}
 
 template <int N, int MODE, int ACC>
 __global__ __launch_bounds__(halo_block) void k_tile4(
@@ -1522,7 +1225,6 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
   const int R = RX * RY;
   // LDS: accumulators (16 bytes per region pixel), then the segment tables
   double* acc64 = smem;                                // acc_f64: re[R], im[R]
-  float* acc32 = reinterpret_cast<float*>(smem);       // acc_f32_rtn: hr, hi, lr, li
   unsigned long long* pairs = reinterpret_cast<unsigned long long*>(smem);  // cas: re[R], im[R]
   unsigned* seg_pref = reinterpret_cast<unsigned*>(smem + 2 * R);
   unsigned* seg_beg = seg_pref + nbins + 1;
@@ -1700,22 +1402,6 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
             __hip_atomic_fetch_add(acc64 + R + li[m], d_i[m], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
           }
-        } else if constexpr (ACC == acc_f32_rtn) {
-#pragma unroll
-          for (int m = 0; m < M; ++m) {
-            if (!ok[m]) continue;
-            const float orr = __hip_atomic_fetch_add(
-              acc32 + li[m], h_r[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            const float ori = __hip_atomic_fetch_add(
-              acc32 + R + li[m], h_i[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            float sum, er, ei;
-            two_sum(orr, h_r[m], sum, er);
-            two_sum(ori, h_i[m], sum, ei);
-            __hip_atomic_fetch_add(acc32 + 2 * R + li[m], er + l_r[m], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(acc32 + 3 * R + li[m], ei + l_i[m], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-          }
         } else {
           // pixel li holds (H_re, L_re) at pairs[li], (H_im, L_im) at pairs[R + li]
           constexpr bool has_l = N > 1;
@@ -1797,18 +1483,11 @@ __global__ __launch_bounds__(halo_block) void k_tile4(
       vr = acc64[k];
       vi = acc64[R + k];
     } else {
-      if constexpr (ACC == acc_f32_rtn) {
-        fr = acc32[k];
-        fi = acc32[R + k];
-        flr = acc32[2 * R + k];
-        fli = acc32[3 * R + k];
-      } else {
-        const unsigned long long pr = pairs[k], pi = pairs[R + k];
-        fr = __uint_as_float(unsigned(pr));
-        flr = __uint_as_float(unsigned(pr >> 32));
-        fi = __uint_as_float(unsigned(pi));
-        fli = __uint_as_float(unsigned(pi >> 32));
-      }
+      const unsigned long long pr = pairs[k], pi = pairs[R + k];
+      fr = __uint_as_float(unsigned(pr));
+      flr = __uint_as_float(unsigned(pr >> 32));
+      fi = __uint_as_float(unsigned(pi));
+      fli = __uint_as_float(unsigned(pi >> 32));
       vr = double(fr) + double(flr);
       vi = double(fi) + double(fli);
     }
@@ -3128,29 +2807,6 @@ size_t tiles_per_footprint(int max_sx, int max_sy) {
   return tx * ty;
 }
 
-hipError_t launch_prep(
-  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
-  const Mueller& mu, bool update_weights, bool do_grid, bool gvis_ready,
-  GvisSource src, const double2* residuals, CallBuffers& cb, double* weights) {
-  if (n == 0) return hipSuccess;
-  const int adj = src == GvisSource::Adjoint;
-#define HPGX_PREP(NP)                                                          \
-  hipLaunchKernelGGL(                                                          \
-    k_prep<NP>, dim3(blocks_for(n)), dim3(block), 0, s,                        \
-    static_cast<const VisRec<NP>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,    \
-    int(update_weights), int(do_grid), int(gvis_ready), adj, residuals,        \
-    cb.prep, cb.gvis, cb.hist, weights, cb.invalid_count)
-  switch (npol) {
-  case 1: HPGX_PREP(1); break;
-  case 2: HPGX_PREP(2); break;
-  case 3: HPGX_PREP(3); break;
-  case 4: HPGX_PREP(4); break;
-  default: return hipErrorInvalidValue;
-  }
-#undef HPGX_PREP
-  return hipGetLastError();
-}
-
 size_t scan_temp_bytes(size_t nkeys) {
   size_t bytes = 0;
   (void)hipcub::DeviceScan::ExclusiveSum(
@@ -3166,55 +2822,6 @@ hipError_t launch_scan(hipStream_t s, size_t nkeys, CallBuffers& cb) {
   if (e != hipSuccess) return e;
   return hipMemcpyAsync(
     cb.cursor, cb.offsets, nkeys * sizeof(unsigned), hipMemcpyDeviceToDevice, s);
-}
-
-hipError_t launch_scatter(
-  hipStream_t s, int npol, size_t n, const Geometry& geo, const Mueller& mu,
-  CallBuffers& cb, const GroupInfo* groups) {
-  if (n == 0) return hipSuccess;
-#define HPGX_SCATTER(NP)                                                       \
-  hipLaunchKernelGGL(                                                          \
-    k_scatter<NP>, dim3(blocks_for(n)), dim3(block), 0, s, n, geo, groups, mu, \
-    cb.prep, cb.cursor, cb.entries)
-  switch (npol) {
-  case 1: HPGX_SCATTER(1); break;
-  case 2: HPGX_SCATTER(2); break;
-  case 3: HPGX_SCATTER(3); break;
-  case 4: HPGX_SCATTER(4); break;
-  default: return hipErrorInvalidValue;
-  }
-#undef HPGX_SCATTER
-  return hipGetLastError();
-}
-
-hipError_t launch_tile_grid(
-  hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
-  const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid) {
-  if (nkeys == 0) return hipSuccess;
-#define HPGX_TILE(NP, MD)                                                      \
-  hipLaunchKernelGGL(                                                          \
-    (k_tile<NP, MD>), dim3(unsigned(nkeys)), dim3(block), 0, s, geo,           \
-    cf.groups, cf.taps, mu, cb.offsets, cb.entries, cb.prep, cb.gvis,          \
-    static_cast<const VisRec<NP>*>(cb.vis), grid)
-  if (mode == 0) {
-    switch (npol) {
-    case 1: HPGX_TILE(1, 0); break;
-    case 2: HPGX_TILE(2, 0); break;
-    case 3: HPGX_TILE(3, 0); break;
-    case 4: HPGX_TILE(4, 0); break;
-    default: return hipErrorInvalidValue;
-    }
-  } else {
-    switch (npol) {
-    case 1: HPGX_TILE(1, 1); break;
-    case 2: HPGX_TILE(2, 1); break;
-    case 3: HPGX_TILE(3, 1); break;
-    case 4: HPGX_TILE(4, 1); break;
-    default: return hipErrorInvalidValue;
-    }
-  }
-#undef HPGX_TILE
-  return hipGetLastError();
 }
 
 hipError_t launch_count(
@@ -3460,8 +3067,6 @@ hipError_t launch_tile_halo(
     HPGX_TILE4_NP(0, acc_f64)
   } else if (acc == 0) {
     HPGX_TILE4_NP(1, acc_f64)
-  } else if (acc == 1) {
-    HPGX_TILE4_NP(1, acc_f32_rtn)
   } else {
     HPGX_TILE4_NP(1, acc_f32_cas)
   }

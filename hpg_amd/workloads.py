@@ -165,3 +165,128 @@ def point_source_image(size, npol, num_sources=200, noise=1e-4, seed=11):
     for p in range(npol):
         img[p, ys, xs] += flux
     return img
+
+
+# ---- device-side generators for the full-size workloads ---------------------
+#
+# The full-size multi-polarization workloads hold 6-50 GB of CF taps and up to
+# 5e7 records; they are generated directly in GPU memory with torch (fixed
+# seeds) and handed to the state through the device routes of the C ABI.
+
+def _torch():
+    import torch
+    return torch
+
+
+def device_records(num_vis, npol, device="cuda"):
+    """(records, fields): a zeroed torch uint8 tensor of num_vis
+    VisData<npol> records on the GPU, and a dict of typed views of its
+    fields (vis_re/vis_im/weights: float32 (n, npol); frequency, phase:
+    float64 (n,); u, v, w: float64 (n,); grid_cube, cf_cube, cf_group: int32
+    (n,); grad_x, grad_y: float64 (n,))."""
+    torch = _torch()
+    dt = visdata_dtype(npol)
+    size = dt.itemsize
+    rec = torch.zeros((num_vis, size), dtype=torch.uint8, device=device)
+
+    def field(off, nbytes, tdt):
+        return rec[:, off:off + nbytes].view(tdt)
+
+    off_v = dt.fields["visibilities"][1]
+    off_w = dt.fields["weights"][1]
+    vis = field(off_v, 8 * npol, torch.float32)  # (n, 2 npol): re, im interleaved
+    f = {
+        "vis": vis,
+        "weights": field(off_w, 4 * npol, torch.float32),
+        "frequency": field(dt.fields["frequency"][1], 8, torch.float64)[:, 0],
+        "phase": field(dt.fields["phase"][1], 8, torch.float64)[:, 0],
+        "uvw": field(dt.fields["uvw"][1], 24, torch.float64),
+        "grid_cube": field(dt.fields["grid_cube"][1], 4, torch.int32)[:, 0],
+        "cf_index": field(dt.fields["cf_index"][1], 8, torch.int32),
+        "gradient": field(dt.fields["cf_phase_gradient"][1], 16, torch.float64),
+    }
+    return rec, f
+
+
+def gaussian_uv_device_records(num_vis, npol, grid_size, group_of_w, sigma_frac,
+                               w_max=1.0, weight_range=(0.5, 1.5), ncube=1, cf_ncube=1,
+                               seed=5, device="cuda"):
+    """VisData<npol> records in GPU memory: values N(0, 1), weights uniform
+    in weight_range, u, v ~ N(0, (sigma_frac * extent)^2) pixels (grid_scale
+    1, frequency c), w ~ U(0, w_max), CF group group_of_w(w) (a torch
+    function of the w tensor), grid cube uniform in [0, ncube), CF cube the
+    grid cube when the CF has one cube per channel (cf_ncube > 1), else 0."""
+    torch = _torch()
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    rec, f = device_records(num_vis, npol, device)
+    f["vis"].copy_(torch.randn((num_vis, 2 * npol), generator=g, device=device))
+    lo, hi = weight_range
+    f["weights"].copy_(lo + (hi - lo) * torch.rand((num_vis, npol), generator=g, device=device))
+    f["frequency"].fill_(C_LIGHT)
+    uvw = torch.empty((num_vis, 3), dtype=torch.float64, device=device)
+    uvw[:, 0] = torch.randn(num_vis, generator=g, device=device, dtype=torch.float64) \
+        * (sigma_frac * grid_size[0])
+    uvw[:, 1] = torch.randn(num_vis, generator=g, device=device, dtype=torch.float64) \
+        * (sigma_frac * grid_size[1])
+    w = torch.rand(num_vis, generator=g, device=device, dtype=torch.float64) * w_max
+    uvw[:, 2] = w
+    f["uvw"].copy_(uvw)
+    grp = group_of_w(w).to(torch.int32)
+    cube = torch.randint(0, ncube, (num_vis,), generator=g, device=device, dtype=torch.int32)
+    f["grid_cube"].copy_(cube)
+    f["cf_index"][:, 0] = cube if cf_ncube > 1 else 0
+    f["cf_index"][:, 1] = grp
+    return rec
+
+
+def taper_chirp_cf_device(oversampling, radii, chirps, nmueller, ncube=1,
+                          phase_screen=False, seed=3, sigma_div=2.5, device="cuda"):
+    """Device (torch complex64) CF groups: group g has support radius
+    radii[g] and taps exp(-rho^2 / (2 (r/sigma_div)^2)) * exp(i pi chirps[g]
+    rho^2) / (1 + m + c/2) (rho in pixels from the centre tap, m the Mueller
+    element, c the cube); with phase_screen each tap is also multiplied by a
+    random unit-modulus factor exp(i phi), phi ~ U(-pi, pi) (seed + g)."""
+    torch = _torch()
+    groups = []
+    for gi, (r, kappa) in enumerate(zip(radii, chirps)):
+        ext = cf_extent(r, oversampling)
+        centre = (r + PADDING) * oversampling
+        t = (torch.arange(ext, device=device, dtype=torch.float64) - centre) / oversampling
+        rho2 = t[None, :] ** 2 + t[:, None] ** 2  # [y][x]
+        sigma = max(r, 1) / sigma_div
+        base = torch.exp(-rho2 / (2.0 * sigma * sigma)) \
+            * torch.exp(1j * np.pi * kappa * rho2)
+        scale = torch.tensor([[1.0 / (1.0 + m + 0.5 * c) for m in range(nmueller)]
+                              for c in range(ncube)], dtype=torch.float64, device=device)
+        vals = (base[None, None] * scale[:, :, None, None]).to(torch.complex64)
+        if phase_screen:
+            gen = torch.Generator(device=device)
+            gen.manual_seed(int(seed) * 100003 + gi)
+            phi = (torch.rand(vals.shape, generator=gen, device=device) * 2.0 - 1.0) * np.pi
+            vals = vals * torch.polar(torch.ones_like(phi), phi)
+        groups.append(vals.contiguous())
+    return CFArray(oversampling, groups)
+
+
+def config2_geometry(ngroups=64, oversampling=20):
+    """Full-pol W-projection workload: CF group g has radius
+    4 + round(28 g / 63) (supports 9..65 pixels) and chirp coefficient
+    0.02 (g + 1/2) / ngroups; records with w in [0, 1) use group
+    floor(ngroups w)."""
+    radii = [4 + int(round(28 * g / max(ngroups - 1, 1))) for g in range(ngroups)]
+    chirps = [0.02 * (g + 0.5) / ngroups for g in range(ngroups)]
+    return radii, chirps
+
+
+def config5_geometry(nw=32, npa=16):
+    """A/W-projection workload: 32 w bins x 16 parallactic-angle bins,
+    group g = wbin * npa + pa; radius 8 + round(56 wbin / 31) (supports
+    17..129 pixels), chirp 0.004 (wbin + 1/2) / nw; records with w in [0, 1)
+    fall in w bin floor(nw w)."""
+    radii, chirps = [], []
+    for wb in range(nw):
+        for _ in range(npa):
+            radii.append(8 + int(round(56 * wb / max(nw - 1, 1))))
+            chirps.append(0.004 * (wb + 0.5) / nw)
+    return radii, chirps

@@ -135,12 +135,12 @@ def test_random_vs_oracle_fp32_compensated(hpg, oracle_mod, case):
     np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
 
 
-@pytest.mark.parametrize("gridder", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("gridder", [2, 3, 4, 5])
 @pytest.mark.parametrize("case", [0, 2, 3, 4])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_each_gridder_vs_oracle(hpg, oracle_mod, monkeypatch, gridder, case, mode):
     """Every gridding pipeline (HPGX_GRIDDER, read when a state is created:
-    1 indexed records, 2 globally binned entries, 3 block-local bins with
+    2 globally binned entries, 3 block-local bins with
     clipped tiles, 4 block-local bins with haloed tiles updated by LDS
     atomics -- which falls back to 3 for case 3, whose 35-pixel support
     exceeds a tile -- and 5 the haloed tiles added per quadrant wave, one
@@ -211,13 +211,12 @@ def test_dense_tile_vs_oracle(hpg, oracle_mod, monkeypatch, gridder, mode):
 
 
 @pytest.mark.parametrize("gridder", [4, 5])
-@pytest.mark.parametrize("scheme", ["cas", "rtn", "fp64"])
+@pytest.mark.parametrize("scheme", ["cas", "fp64"])
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_fp32_tile_schemes(hpg, oracle_mod, monkeypatch, gridder, scheme, case):
     """Each tile accumulation of the FP32 grid (HPGX_FP32_TILE: FP32 two-sum
     pairs -- under compare-and-swap in gridder 4, plain read-modify-write in
-    gridder 5 --, FP32 two-sum through returning atomics (gridder 4), FP64
-    tile sums) against the FP64 oracle: e_max, e_rms <= 1e-6 per plane.  The
+    gridder 5 --, or FP64 tile sums) against the FP64 oracle: e_max, e_rms <= 1e-6 per plane.  The
     records are dense (3000 on grids of 40-70 pixels), so pixels collect
     hundreds of contributions and lanes collide on pixels."""
     from hpg_amd import workloads
