@@ -92,10 +92,15 @@ def test_finite_differences_via_autograd(hpg):
         p = gs.degrid_forward(rec)
         return float((wts * np.abs(p - data) ** 2).sum())
 
+    # the loss is quadratic in the model, so central differences are exact
+    # up to rounding: every checked pixel meets a per-pixel relative 1e-4.
+    # Pixels no record touches have a zero gradient; the 64 pixels are drawn
+    # from those the footprints cover.
     h = 1e-3
-    flat = rng.choice(model.size, 64, replace=False)
     gmax = np.abs(grad).max()
-    for k in flat:
+    touched = np.flatnonzero(np.abs(grad).ravel() > 1e-3 * gmax)
+    assert touched.size >= 64
+    for k in rng.choice(touched, 64, replace=False):
         idx = np.unravel_index(k, model.shape)
         fd = []
         for unit in (1.0, 1j):
@@ -105,7 +110,7 @@ def test_finite_differences_via_autograd(hpg):
             mm[idx] -= h * unit
             fd.append((L(mp) - L(mm)) / (2 * h))
         g_fd = fd[0] + 1j * fd[1]
-        assert abs(g_fd - grad[idx]) <= 1e-4 * gmax, (idx, g_fd, grad[idx])
+        assert abs(g_fd - grad[idx]) <= 1e-4 * abs(grad[idx]), (idx, g_fd, grad[idx])
 
 
 def test_adjoint_is_gridding_of_scaled_residuals(hpg):

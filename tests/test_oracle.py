@@ -453,3 +453,140 @@ def test_openmp_baseline_matches_serial(oracle_mod):
     np.testing.assert_allclose(out[1][0], out[0][0], rtol=0,
                                atol=1e-12 * np.abs(out[0][0]).max())
     np.testing.assert_allclose(out[1][1], out[0][1], rtol=1e-12)
+
+
+def _adjoint_case(oracle_mod, M, Mc, seed):
+    """Random complex CFs (2 groups, 2 cubes), 2 visibility polarizations,
+    2 grid rows, records with both signs of w, random phases and phase
+    gradients."""
+    rng = np.random.default_rng(seed)
+    os_ = 4
+    npol = 2
+    gs = (24, 20, 2, 2)
+    groups = []
+    for r in (2, 3):
+        e = (2 * r + 1 + 4) * os_
+        v = rng.uniform(-1, 1, (2, 4, e, e, 2)).astype(np.float32)
+        groups.append(((e, e, 4, 2), (v[..., 0] + 1j * v[..., 1]).astype(np.complex64)))
+    dt = oracle_mod.visdata_dtype(npol)
+    n = 24
+    rec = np.zeros(n, dtype=dt)
+    rec["weights"] = 1.0
+    rec["frequency"] = rc.C_LIGHT
+    rec["phase"] = rng.uniform(-np.pi, np.pi, n)
+    rec["uvw"][:, 0] = rng.uniform(-6, 6, n)
+    rec["uvw"][:, 1] = rng.uniform(-5, 5, n)
+    rec["uvw"][:, 2] = np.where(np.arange(n) % 2 == 0, 1.0, -1.0) * rng.uniform(0.1, 1, n)
+    rec["cf_index"][:, 1] = rng.integers(0, 2, n)
+    rec["cf_index"][:, 0] = rng.integers(0, 2, n)
+    rec["grid_cube"] = rng.integers(0, 2, n)
+    rec["cf_phase_gradient"] = rng.uniform(-0.3, 0.3, (n, 2))
+    return gs, os_, groups, rec, rng
+
+
+@pytest.mark.parametrize("conj_differs", [False, True])
+def test_degrid_adjoint_identity(oracle_mod, conj_differs):
+    """Pins the oracle's degridder (degrid_vis, src/hpg_core.hpp:579-682)
+    against its gridder (grid_vis, 791-886) on the CPU: for every record,
+
+        <degrid_{M,Mc}(m), r> = <m, grid_{Mc,M}(r phasor / conj(W))>
+
+    with W_v the degridding normalization.  degrid_vis reads Mueller row
+    (w > 0 ? Mc : M) and conj(CF) relative to grid_vis, which reads
+    (w > 0 ? M : Mc), so the adjoint of degridding with (M, Mc) is gridding
+    with the roles of M and Mc exchanged.  W_v is not taken from the
+    degridder: it is the conjugated sum over grid rows and footprint of the
+    gridder's own output for a unit value in polarization v (phase 0, no
+    phase gradient), so a wrong conjugation, Mueller row, phase screen sign,
+    phasor or normalization in either function breaks the identity.  Both
+    sides hold FP32-rounded values (predictions, gvis, CF products):
+    tolerance 1e-5 of |degrid(m)| |r|."""
+    M = [[0, 1], [2, 3]]
+    Mc = [[1, 0], [3, 2]] if conj_differs else M
+    gs, os_, groups, rec, rng = _adjoint_case(oracle_mod, M, Mc, 90 + int(conj_differs))
+    shape = (gs[3], gs[2], gs[1], gs[0])
+    model = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    checked = 0
+    for i in range(rec.shape[0]):
+        one = rec[i:i + 1].copy()
+        # degridding normalization from the gridder with the exchanged rows
+        W = np.zeros(2, dtype=np.complex128)
+        for v in range(2):
+            unit = one.copy()
+            unit["vis"][0, :] = 0
+            unit["vis"][0, v] = 1.0
+            unit["phase"] = 0.0
+            unit["cf_phase_gradient"] = 0.0
+            og = oracle_mod.Oracle(gs, (1.0, 1.0), Mc, M)
+            og.set_cf(os_, groups)
+            og.grid_visibilities(unit, False)
+            W[v] = np.conj(og.grid_values().sum())
+        if not np.all(np.abs(W) > 0):
+            continue  # record off the grid
+        od = oracle_mod.Oracle(gs, (1.0, 1.0), M, Mc)
+        od.set_cf(os_, groups)
+        od.set_model(model)
+        pred = od.grid_visibilities(one, False, do_degrid=True, do_grid=False)["vis"][0]
+        r = (rng.standard_normal(2) + 1j * rng.standard_normal(2))
+        lhs = np.vdot(pred.astype(np.complex128), r)
+        adj = one.copy()
+        adj["vis"][0, :] = (r / np.conj(W)).astype(np.complex64)
+        oa = oracle_mod.Oracle(gs, (1.0, 1.0), Mc, M)
+        oa.set_cf(os_, groups)
+        oa.grid_visibilities(adj, False)
+        rhs = np.vdot(model, oa.grid_values())
+        scale = np.linalg.norm(pred) * np.linalg.norm(r)
+        assert abs(lhs - rhs) <= 1e-5 * scale, (i, lhs, rhs)
+        checked += 1
+    assert checked >= 10
+
+
+def test_degrid_adjoint_identity_detects_planted_errors(oracle_mod):
+    """The identity above fails for plausible mistakes: predictions that
+    miss the phasor, or a model read through the wrong phase-screen sign
+    (degridding the conjugated-gradient record)."""
+    M = [[0, 1], [2, 3]]
+    gs, os_, groups, rec, rng = _adjoint_case(oracle_mod, M, M, 97)
+    shape = (gs[3], gs[2], gs[1], gs[0])
+    model = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    bad_phasor = bad_screen = 0
+    for i in range(rec.shape[0]):
+        one = rec[i:i + 1].copy()
+        W = np.zeros(2, dtype=np.complex128)
+        for v in range(2):
+            unit = one.copy()
+            unit["vis"][0, :] = 0
+            unit["vis"][0, v] = 1.0
+            unit["phase"] = 0.0
+            unit["cf_phase_gradient"] = 0.0
+            og = oracle_mod.Oracle(gs, (1.0, 1.0), M, M)
+            og.set_cf(os_, groups)
+            og.grid_visibilities(unit, False)
+            W[v] = np.conj(og.grid_values().sum())
+        if not np.all(np.abs(W) > 0):
+            continue
+
+        def degrid(rec1):
+            od = oracle_mod.Oracle(gs, (1.0, 1.0), M, M)
+            od.set_cf(os_, groups)
+            od.set_model(model)
+            return od.grid_visibilities(rec1, False, do_degrid=True,
+                                        do_grid=False)["vis"][0].astype(np.complex128)
+
+        r = rng.standard_normal(2) + 1j * rng.standard_normal(2)
+        adj = one.copy()
+        adj["vis"][0, :] = (r / np.conj(W)).astype(np.complex64)
+        oa = oracle_mod.Oracle(gs, (1.0, 1.0), M, M)
+        oa.set_cf(os_, groups)
+        oa.grid_visibilities(adj, False)
+        rhs = np.vdot(model, oa.grid_values())
+        pred = degrid(one)
+        scale = np.linalg.norm(pred) * np.linalg.norm(r)
+        phasor = np.exp(1j * one["phase"][0])
+        if abs(np.vdot(pred * phasor, r) - rhs) > 1e-3 * scale:
+            bad_phasor += 1
+        flipped = one.copy()
+        flipped["cf_phase_gradient"] *= -1
+        if abs(np.vdot(degrid(flipped), r) - rhs) > 1e-3 * scale:
+            bad_screen += 1
+    assert bad_phasor >= 8 and bad_screen >= 8
