@@ -35,6 +35,7 @@ SIGNATURES = [
     ("hpgx_reset_model", i, [vp]),
     ("hpgx_has_model", i, [vp]),
     ("hpgx_grid_visibilities", i, [vp, vp, sz, u, i, i, i, i, vp, i]),
+    ("hpgx_grid_visibilities_noalloc", i, [vp, vp, sz, u, i]),
     ("hpgx_degrid_forward", i, [vp, vp, vp, sz, u, i]),
     ("hpgx_degrid_adjoint", i, [vp, vp, vp, sz, u, i]),
     ("hpgx_fence", i, [vp]),

@@ -43,6 +43,42 @@ struct Plan {
 
 }  // namespace
 
+// Fixed device scratch of hpgx_grid_visibilities_noalloc: the bin offset
+// table, bin entries, entry record indexes and halo values of its gridding
+// passes.  Each GPU the library runs on holds its own copy until the process
+// ends.  Host code never uses the array's name as an address: the device
+// address of each GPU's copy comes from hipGetSymbolAddress (noalloc_scratch).
+constexpr size_t noalloc_scratch_bytes = size_t(HPGX_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[noalloc_scratch_bytes];
+
+namespace {
+constexpr int noalloc_max_devices = 64;
+std::mutex g_noalloc_mtx;
+void* g_noalloc_ptr[noalloc_max_devices] = {};
+
+// device address of `device`'s scratch, looked up once per device with that
+// device current
+hipError_t noalloc_scratch(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= noalloc_max_devices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_noalloc_mtx);
+  if (!g_noalloc_ptr[device]) {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_noalloc_ptr[device] = p;
+  }
+  *out = g_noalloc_ptr[device];
+  return hipSuccess;
+}
+}  // namespace
+
 struct hpgx_state_s {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -596,11 +632,13 @@ int run_degrid_binned(hpgx_state_t st, size_t n, bool do_grid, const void* recor
 // records: device records read in place (nullptr: the staged copy cb.vis);
 // out_records: where the binned degridder writes predicted / residual
 // values when records are read in place (nullptr: not written)
-int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
-             bool do_grid, hpgx::GvisSource src, const double2* residuals,
-             const void* records = nullptr, void* out_records = nullptr) {
+// cb: the per-call buffers (the state's own, or views into the fixed
+// scratch of hpgx_grid_visibilities_noalloc); pass_blocks: bin blocks per
+// pass of the block-local pipelines
+int run_grid_with(hpgx_state_t st, CallBuffers& cb, size_t pass_blocks, size_t n,
+                  bool update_weights, bool do_degrid, bool do_grid, hpgx::GvisSource src,
+                  const double2* residuals, const void* records, void* out_records) {
   const Geometry geo = st->geometry();
-  CallBuffers& cb = st->cb;
   void* rec_base = records ? const_cast<void*>(records) : cb.vis;
   const int np = int(st->npol);
   bool gvis_ready = false;
@@ -621,13 +659,13 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
     const size_t block_cap = hpgx::bin_block_records() * epr;
     // bin blocks per pass as sized by ensure_call_buffers (entries, offset
     // table and the tile kernel's LDS segment tables)
-    const size_t chunk = std::max<size_t>(1, st->pass_blocks) * hpgx::bin_block_records();
+    const size_t chunk = std::max<size_t>(1, pass_blocks) * hpgx::bin_block_records();
     const size_t rec_bytes = visdata_size(st->npol);
     for (size_t off = 0; off < n; off += chunk) {
       const size_t m = std::min(chunk, n - off);
       CallBuffers c = cb;
       c.vis = static_cast<char*>(rec_base) + off * rec_bytes;
-      c.gvis = cb.gvis + off * np;
+      c.gvis = cb.gvis ? cb.gvis + off * np : nullptr;
       const double2* res = residuals ? residuals + off * np : nullptr;
       HPGX_HIP(st, hpgx::launch_bin(
                      st->stream, np, m, geo, st->cf, st->mu, update_weights, gvis_ready, src,
@@ -670,6 +708,75 @@ int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
   return HPGX_SUCCESS;
 }
 
+int run_grid(hpgx_state_t st, size_t n, bool update_weights, bool do_degrid,
+             bool do_grid, hpgx::GvisSource src, const double2* residuals,
+             const void* records = nullptr, void* out_records = nullptr) {
+  return run_grid_with(st, st->cb, st->pass_blocks, n, update_weights, do_degrid, do_grid, src,
+                       residuals, records, out_records);
+}
+
+// Views into the fixed scratch for gridding n records without allocating:
+// bin blocks per pass as many as fit (at most what one pass of the
+// pipeline takes), the records gridded in as many passes as needed.
+// Nothing in these buffers has to start at zero: k_bin writes each offset
+// table row and entry it produces before the tile kernel reads them, and
+// the tile kernel writes every key's halo flag before k_halo reads it.
+int noalloc_plan(hpgx_state_t st, void* base, size_t n, CallBuffers& c, size_t& pass_blocks) {
+  if (!use_block_local(st))
+    return fail(st, HPGX_ERR_OTHER,
+                "hpgx_grid_visibilities_noalloc: grids of more than "
+                  + std::to_string(hpgx::bin_keys_limit())
+                  + " (plane, 32x32 tile) keys are not supported (this grid has "
+                  + std::to_string(st->nkeys) + ")");
+  const bool halo = use_halo(st);
+  const size_t epr = entries_per_record(st);
+  const size_t block_cap = hpgx::bin_block_records() * epr;
+  const size_t esize =
+    halo ? hpgx::halo_entry_bytes(int(st->npol)) : hpgx::entry_bytes(int(st->npol));
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t halo_bytes =
+    halo ? up(st->nkeys * hpgx::halo_pixels_per_key(halo_x(st), halo_y(st)) * sizeof(double2))
+    : 0;
+  const size_t set_bytes = halo ? up(st->nkeys * sizeof(unsigned)) : 0;
+  const size_t per_block =
+    (st->nkeys + 1) * sizeof(unsigned) + block_cap * esize + block_cap * sizeof(unsigned);
+  const size_t fixed = halo_bytes + set_bytes + 3 * 256;
+  const size_t fit = noalloc_scratch_bytes > fixed ? (noalloc_scratch_bytes - fixed) / per_block : 0;
+  const size_t bins_max = halo ? halo_bins_limit(st) : size_t(hpgx::tile_bins_limit());
+  const size_t needed = (n + hpgx::bin_block_records() - 1) / hpgx::bin_block_records();
+  const size_t blocks = std::min({needed, bins_max, fit});
+  if (blocks == 0)
+    return fail(st, HPGX_ERR_OTHER,
+                "hpgx_grid_visibilities_noalloc: one bin block of this grid and CF needs "
+                  + std::to_string(fixed + per_block) + " bytes of scratch; the fixed scratch holds "
+                  + std::to_string(noalloc_scratch_bytes));
+  char* p = static_cast<char*>(base);
+  c = CallBuffers{};
+  c.off_table = reinterpret_cast<unsigned*>(p);
+  c.off_table_capacity = blocks * (st->nkeys + 1);
+  p += up(c.off_table_capacity * sizeof(unsigned));
+  c.stream_entries = p;
+  c.entries_capacity = blocks * block_cap;
+  p += up(c.entries_capacity * esize);
+  c.entry_vis = reinterpret_cast<unsigned*>(p);
+  p += up(c.entries_capacity * sizeof(unsigned));
+  if (halo) {
+    c.halo = reinterpret_cast<double2*>(p);
+    c.halo_capacity = st->nkeys * hpgx::halo_pixels_per_key(halo_x(st), halo_y(st));
+    p += halo_bytes;
+    c.halo_set = reinterpret_cast<unsigned*>(p);
+    c.halo_set_capacity = st->nkeys;
+    p += set_bytes;
+  }
+  if (size_t(p - static_cast<char*>(base)) > noalloc_scratch_bytes)
+    return fail(st, HPGX_ERR_OTHER, "hpgx_grid_visibilities_noalloc: scratch layout overflow");
+  // skipped records are counted in the state's own counter (read by
+  // hpgx_fence), as by hpgx_grid_visibilities
+  c.invalid_count = st->cb.invalid_count;
+  pass_blocks = blocks;
+  return HPGX_SUCCESS;
+}
+
 int stage_records(hpgx_state_t st, const void* vis, size_t n, int memory) {
   const size_t bytes = n * visdata_size(st->npol);
   if (bytes == 0) return HPGX_SUCCESS;
@@ -703,6 +810,14 @@ int hpgx_initialize(int device_id) {
     int d = 0;
     (void)hipGetDevice(&d);
     g_device = d;
+  }
+  void* p = nullptr;
+  const hipError_t e = noalloc_scratch(g_device, &p);
+  if (e != hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_create_mtx);
+    g_create_error = std::string("scratch address of the allocation-free entry point: ")
+      + hipGetErrorString(e);
+    return HPGX_ERR_OTHER;
   }
   return HPGX_SUCCESS;
 }
@@ -1106,6 +1221,30 @@ int hpgx_grid_visibilities(
                    st->stream));
   }
   return HPGX_SUCCESS;
+}
+
+int hpgx_grid_visibilities_noalloc(
+  hpgx_state_t st, const void* vis, size_t n, unsigned npol, int update_grid_weights) {
+  if (!st) return HPGX_ERR_OTHER;
+  // validation of hpgx_grid_visibilities, in the same order
+  if (n > st->max_batch)
+    return fail(st, HPGX_ERR_EXCESSIVE_NUMBER_VISIBILITIES,
+                "Number of visibilities exceeds maximum batch size");
+  if (npol != st->npol)
+    return fail(st, HPGX_ERR_INVALID_NUMBER_POLARIZATIONS,
+                "Number of visibility polarizations does not match Mueller matrix");
+  if (n == 0) return HPGX_SUCCESS;
+  if (!vis) return fail(st, HPGX_ERR_OTHER, "null visibilities");
+  if (int rc = set_device(st)) return rc;
+  // no CF: every record refers to a missing CF group and is skipped
+  if (st->cf.ngroups == 0) return HPGX_SUCCESS;
+  void* base = nullptr;
+  HPGX_HIP(st, noalloc_scratch(st->device, &base));
+  CallBuffers c{};
+  size_t pass_blocks = 0;
+  if (int rc = noalloc_plan(st, base, n, c, pass_blocks)) return rc;
+  return run_grid_with(st, c, pass_blocks, n, update_grid_weights != 0, false, true,
+                       hpgx::GvisSource::Values, nullptr, vis, nullptr);
 }
 
 int hpgx_degrid_adjoint(

@@ -191,6 +191,32 @@ HPGX_API int hpgx_grid_visibilities(
   unsigned num_polarizations, int update_grid_weights, int do_degrid,
   int return_visibilities, int do_grid, void* visibilities_out, int memory);
 
+/* Bytes of the fixed device scratch of hpgx_grid_visibilities_noalloc, held
+ * by every GPU the library runs on from the library's load until the process
+ * ends. */
+#define HPGX_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* Allocation-free form of hpgx_grid_visibilities for device-resident records
+ * (HPGX_DEVICE memory): gridding only, i.e. do_degrid = 0,
+ * return_visibilities = 0, do_grid = 1, with the same validation, results and
+ * skipped-record reporting (hpgx_fence).  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  It calls no
+ * hipMalloc, hipFree, hipMemset or allocating library routine: its bin
+ * entries, offset table and halo values live in a fixed
+ * HPGX_NOALLOC_SCRATCH_BYTES device array of the library, and a batch whose
+ * bins exceed it is gridded in several passes.  Grids of more than 8192
+ * (plane, 32x32 tile) keys, or a grid and CF for which one bin block of 4096
+ * records does not fit the scratch, are rejected with HPGX_ERR_OTHER and a
+ * message.  The records must be ready when the call is made and stay
+ * unchanged until the next hpgx_fence.  Every call on one GPU uses that GPU's
+ * one scratch array: calls that may overlap in time on the same GPU (states
+ * on different streams, or several host threads) race and must be ordered by
+ * the caller; calls on one state are ordered by its stream. */
+HPGX_API int hpgx_grid_visibilities_noalloc(
+  hpgx_state_t state, const void* visibilities, size_t num_visibilities,
+  unsigned num_polarizations, int update_grid_weights);
+
 /* Forward operator of the differentiable form: the degridded (predicted)
  * visibilities of the current model for the records' coordinates, before
  * their rounding to complex<float>: predictions are num_visibilities x
