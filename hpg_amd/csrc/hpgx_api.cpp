@@ -121,6 +121,10 @@ struct hpgx_state_s {
   // hi/lo pairs updated by two-sum (default; under compare-and-swap in
   // gridder 4), 0 = FP64 tile sums
   int fp32_tile = 2;
+  // FP32-compensated grids with supports up to 8 x 8 on the block-local
+  // haloed bins: quadrant-keyed bins and k_sub instead of k_quad
+  // (HPGX_SUB_TILES=0 keeps k_quad)
+  bool sub_tiles = true;
 
   Plan plan_z2z, plan_c2c;
   hipEvent_t events[num_event_slots] = {};
@@ -277,12 +281,34 @@ bool use_quad(hpgx_state_t st) {
     && st->taps_capacity < (size_t(1) << 32);
 }
 
+// the quadrant-key kernel (k_sub): FP32-compensated grids on block-local
+// haloed bins with supports up to 8 x 8, and 4 bin keys per (plane, tile)
+// within one workgroup's LDS histogram
+bool use_sub(hpgx_state_t st) {
+  return st->sub_tiles && use_quad(st) && use_halo(st)
+    && hpgx::sub_supported(halo_x(st), halo_y(st))
+    && st->taps_capacity * sizeof(float2) <= size_t(0xffffffffu)
+    && 4 * st->nkeys <= size_t(hpgx::bin_keys_limit());
+}
+
+// every CF group has the same support
+bool uniform_support(hpgx_state_t st) {
+  for (const GroupInfo& g : st->groups)
+    if (g.sx != st->groups[0].sx || g.sy != st->groups[0].sy) return false;
+  return !st->groups.empty();
+}
+
+// bin keys of the block-local pipelines: (plane, tile), or (plane, tile,
+// quadrant) for k_sub
+size_t bin_keys(hpgx_state_t st) { return use_sub(st) ? 4 * st->nkeys : st->nkeys; }
+
 // bin blocks per haloed-tile pass: the per-block segment tables share the
 // workgroup's 64 KiB of LDS with the tile
 size_t halo_bins_limit(hpgx_state_t st) {
   size_t nb = size_t(hpgx::tile_bins_limit());
   while (nb > 1 && hpgx::halo_lds_bytes(halo_x(st), halo_y(st), int(nb)) > 65536) nb /= 2;
   if (use_quad(st)) nb = std::min(nb, size_t(hpgx::quad_bins_limit()));
+  if (use_sub(st)) nb = std::min(nb, size_t(hpgx::sub_bins_limit()));
   return nb;
 }
 
@@ -340,7 +366,7 @@ int ensure_call_buffers(hpgx_state_t st) {
         std::max<size_t>(1, std::min({blocks_needed, budget / block_cap, bins_max}));
       need = blocks * block_cap;
       st->pass_blocks = blocks;
-      const size_t tab = blocks * (st->nkeys + 1);
+      const size_t tab = blocks * (bin_keys(st) + 1);
       if (tab > cb.off_table_capacity) {
         dev_free(cb.off_table);
         cb.off_table_capacity = 0;
@@ -667,10 +693,16 @@ int run_grid_with(hpgx_state_t st, CallBuffers& cb, size_t pass_blocks, size_t n
       c.vis = static_cast<char*>(rec_base) + off * rec_bytes;
       c.gvis = cb.gvis ? cb.gvis + off * np : nullptr;
       const double2* res = residuals ? residuals + off * np : nullptr;
+      const bool sub = use_sub(st);
       HPGX_HIP(st, hpgx::launch_bin(
                      st->stream, np, m, geo, st->cf, st->mu, update_weights, gvis_ready, src,
-                     res, c, block_cap, st->weights, halo));
-      if (halo)
+                     res, c, block_cap, st->weights, halo, false, nullptr, sub));
+      if (sub)
+        HPGX_HIP(st, hpgx::launch_tile_sub(
+                       st->stream, np, m, st->nkeys, geo, st->cf,
+                       st->taps_capacity * sizeof(float2), uniform_support(st), st->mu, c,
+                       block_cap, halo_x(st), halo_y(st), st->grid));
+      else if (halo)
         HPGX_HIP(st, hpgx::launch_tile_halo(
                        st->stream, np, mode, st->fp32_tile, m, st->nkeys, geo, st->cf, st->mu, c,
                        block_cap, halo_x(st), halo_y(st), st->grid, false, use_quad(st)));
@@ -739,7 +771,7 @@ int noalloc_plan(hpgx_state_t st, void* base, size_t n, CallBuffers& c, size_t& 
     : 0;
   const size_t set_bytes = halo ? up(st->nkeys * sizeof(unsigned)) : 0;
   const size_t per_block =
-    (st->nkeys + 1) * sizeof(unsigned) + block_cap * esize + block_cap * sizeof(unsigned);
+    (bin_keys(st) + 1) * sizeof(unsigned) + block_cap * esize + block_cap * sizeof(unsigned);
   const size_t fixed = halo_bytes + set_bytes + 3 * 256;
   const size_t fit = noalloc_scratch_bytes > fixed ? (noalloc_scratch_bytes - fixed) / per_block : 0;
   const size_t bins_max = halo ? halo_bins_limit(st) : size_t(hpgx::tile_bins_limit());
@@ -753,7 +785,7 @@ int noalloc_plan(hpgx_state_t st, void* base, size_t n, CallBuffers& c, size_t& 
   char* p = static_cast<char*>(base);
   c = CallBuffers{};
   c.off_table = reinterpret_cast<unsigned*>(p);
-  c.off_table_capacity = blocks * (st->nkeys + 1);
+  c.off_table_capacity = blocks * (bin_keys(st) + 1);
   p += up(c.off_table_capacity * sizeof(unsigned));
   c.stream_entries = p;
   c.entries_capacity = blocks * block_cap;
@@ -883,6 +915,7 @@ int hpgx_create(
     const std::string v(a);
     st->fp32_tile = v == "fp64" ? 0 : 2;
   }
+  if (const char* a = std::getenv("HPGX_SUB_TILES")) st->sub_tiles = std::atoi(a) != 0;
   if (const char* g = std::getenv("HPGX_GRIDDER")) {
     const int v = std::atoi(g);
     st->gridder = (v >= 2 && v <= 5) ? v : 5;

@@ -129,7 +129,8 @@ hipError_t launch_bin(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool update_weights, bool gvis_ready, GvisSource src,
   const double2* residuals, CallBuffers& cb, size_t block_cap, double* weights,
-  bool halo_entries, bool degrid_pass = false, void* copy_records = nullptr);
+  bool halo_entries, bool degrid_pass = false, void* copy_records = nullptr,
+  bool quad_keys = false);
 // haloed tiles (one entry per record and plane; supports below tile)
 size_t halo_entry_bytes(int npol);
 size_t halo_pixels_per_key(int hx, int hy);
@@ -147,6 +148,18 @@ hipError_t launch_tile_halo(
   hipStream_t s, int npol, int mode, int acc, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
   int hx, int hy, void* grid, bool global_bins = false, bool quad = false);
+// the quadrant-key kernel (k_sub): FP32-compensated grids, supports of at
+// most 8 x 8 (hx, hy <= 7), bins of launch_bin with quad_keys; ntile_keys
+// (plane, tile) keys, whose offset-table rows hold 4 ntile_keys + 1 values;
+// CF taps of at most 2^32 - 1 bytes
+size_t sub_lds_bytes(int hx, int hy);
+int sub_bins_limit();
+bool sub_supported(int hx, int hy);
+// uni: every CF group has the same support
+hipError_t launch_tile_sub(
+  hipStream_t s, int npol, size_t n, size_t ntile_keys, const Geometry& geo, const DeviceCF& cf,
+  size_t taps_bytes, bool uni, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
+  int hx, int hy, void* grid);
 hipError_t launch_tile_blocal(
   hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,

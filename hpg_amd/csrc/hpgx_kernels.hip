@@ -55,6 +55,8 @@ static_assert(sizeof(VisRec<2>) == 96, "VisData<2>");
 static_assert(sizeof(VisRec<3>) == 112, "VisData<3>");
 static_assert(sizeof(VisRec<4>) == 120, "VisData<4>");
 
+typedef int v2i_t __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -678,6 +680,15 @@ constexpr unsigned e4_gradient = 1u << 29;
 // k_emit<N, true> writes this layout as 2 + 2N 32-bit words
 static_assert(sizeof(Entry4<1>) == 16 && sizeof(Entry4<4>) == 40, "Entry4 layout");
 
+// bin key of tile key k for a support corner (gx, gy): k itself, or with
+// quad_keys 4 k + the quadrant of the corner within its tile
+__device__ __forceinline__ int bin_key(int k, int gx, int gy, int quad_keys) {
+  if (!quad_keys) return k;
+  const int qx = (gx % tile) >= tile / 2 ? 1 : 0;
+  const int qy = (gy % tile) >= tile / 2 ? 1 : 0;
+  return 4 * k + (qx | (qy << 1));
+}
+
 // HALO = false: one entry per (plane, tile) the support touches (Entry<N>,
 // for k_tile3); HALO = true: one entry per plane, keyed by the tile of the
 // support corner (Entry4<N>, for k_tile4)
@@ -689,13 +700,16 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
   const float2* __restrict__ gvis_in, const double2* __restrict__ residuals,
   unsigned* __restrict__ off_table, void* __restrict__ entries_out,
   unsigned* __restrict__ entry_vis, size_t block_cap, double* __restrict__ weights,
-  unsigned long long* __restrict__ invalid, VisRec<N>* __restrict__ copy_recs) {
+  unsigned long long* __restrict__ invalid, VisRec<N>* __restrict__ copy_recs, int quad_keys) {
 
   __shared__ unsigned cnt[bin_keys_max];
   __shared__ double wl[bin_planes_max];
   __shared__ unsigned part[bin_threads / wave + 1];
 
-  const int nkeys = geo.ntx * geo.nty * geo.nr * geo.nc;
+  // quad_keys (HALO only): a key per 16 x 16 quadrant of each (plane, tile),
+  // key = 4 * tile key + quadrant of the support corner, so that the
+  // entries of one quadrant are contiguous within each bin block
+  const int nkeys = geo.ntx * geo.nty * geo.nr * geo.nc * (quad_keys ? 4 : 1);
   const int nplanes = geo.nr * geo.nc;
   const bool local_w = nplanes <= bin_planes_max;
   for (int k = threadIdx.x; k < nkeys; k += bin_threads) cnt[k] = 0;
@@ -832,8 +846,9 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
       const int base = (int(gcube) * geo.nr + gpol) * ntiles;
       for (int ty = ty0; ty < ty1; ++ty)
         for (int tx = tx0; tx < tx1; ++tx)
-          __hip_atomic_fetch_add(cnt + base + ty * geo.ntx + tx, 1u, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(
+            cnt + bin_key(base + ty * geo.ntx + tx, c.gx, c.gy, quad_keys), 1u,
+            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
   }
   // the threads' weight sums: one sum per (wave, grid cube) present, the
@@ -895,7 +910,8 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
       for (int ty = ty0; ty < ty1; ++ty)
         for (int tx = tx0; tx < tx1; ++tx) {
           const unsigned pos = __hip_atomic_fetch_add(
-            cnt + base + ty * geo.ntx + tx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            cnt + bin_key(base + ty * geo.ntx + tx, e[j].h.gx, e[j].h.gy, quad_keys), 1u,
+            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           if constexpr (HALO) {
 #pragma unroll
             for (int r = 0; r < 4; ++r)
@@ -1710,6 +1726,84 @@ __device__ __noinline__ void quad_gradient_entry(
   }
 }
 
+// The flush of the quadrant kernels: the sum of the four quadrant regions
+// of (hi, lo) pairs (LDS, float4 per pixel, quadrant qd at qd * QX * QY)
+// over the tile and its halo.  Tile pixels are added to the FP32 grid with
+// one rounding (a two-sum of the grid value and hi, then hi + (err + lo));
+// halo pixels are stored as FP64 hi + lo into the key's halo values hk.
+// BLOCK threads, PER pixels per thread: BLOCK * PER >= (tile + hx) * (tile + hy).
+template <int BLOCK, int PER>
+__device__ __forceinline__ void quad_flush_tile(
+  const float4* regions, int QX, int QY, int hx, int hy, const Geometry& geo,
+  unsigned plane, int x0, int y0, float2* __restrict__ grid, double2* __restrict__ hk) {
+  const int Q = QX * QY;
+  const int RX = tile + hx;
+  const int RY = tile + hy;
+  const int R = RX * RY;
+  // every pixel's value first, with the grid loads of the tile's pixels
+  // issued together, then the stores
+  float4 val[PER];
+  float2 g0[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int k = int(threadIdx.x) + j * BLOCK;
+    val[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    g0[j] = make_float2(0.0f, 0.0f);
+    if (k >= R) continue;
+    const int ry = k / RX;
+    const int rx = k - ry * RX;
+    // (hi, lo) of the pixel: the quadrants' pairs combined by two-sum
+    float fr = 0.0f, fi = 0.0f, flr = 0.0f, fli = 0.0f;
+    bool first = true;
+#pragma unroll
+    for (int qd = 0; qd < 4; ++qd) {
+      const int ux = rx - (qd & 1) * quad_edge;
+      const int uy = ry - (qd >> 1) * quad_edge;
+      if (ux < 0 || uy < 0 || ux >= QX || uy >= QY) continue;
+      const float4 a = regions[qd * Q + uy * QX + ux];
+      if (first) {
+        fr = a.x;
+        fi = a.y;
+        flr = a.z;
+        fli = a.w;
+      } else {
+        float sum, err;
+        two_sum(fr, a.x, sum, err);
+        fr = sum;
+        flr = flr + (err + a.z);
+        two_sum(fi, a.y, sum, err);
+        fi = sum;
+        fli = fli + (err + a.w);
+      }
+      first = false;
+    }
+    val[j] = make_float4(fr, fi, flr, fli);
+    const int xx = x0 + rx, yy = y0 + ry;
+    if (rx < tile && ry < tile && xx < geo.nx && yy < geo.ny)
+      g0[j] = grid[(size_t(plane) * geo.ny + yy) * geo.nx + xx];
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int k = int(threadIdx.x) + j * BLOCK;
+    if (k >= R) continue;
+    const int ry = k / RX;
+    const int rx = k - ry * RX;
+    const float4 v = val[j];
+    if (rx < tile && ry < tile) {
+      const int xx = x0 + rx, yy = y0 + ry;
+      if (xx >= geo.nx || yy >= geo.ny) continue;
+      float sr, er, si, ei;
+      two_sum(g0[j].x, v.x, sr, er);
+      two_sum(g0[j].y, v.y, si, ei);
+      grid[(size_t(plane) * geo.ny + yy) * geo.nx + xx] =
+        make_float2(sr + (er + v.z), si + (ei + v.w));
+    } else {
+      const int h = ry < tile ? ry * hx + (rx - tile) : tile * hx + (ry - tile) * RX + rx;
+      hk[h] = make_double2(double(v.x) + double(v.z), double(v.y) + double(v.w));
+    }
+  }
+}
+
 // ONE: every support fits one step (sx, sy <= 8); the pipelined loop is
 // then free of branches around vector memory loads, so each entry waits for
 // its own CF taps only and not for the loads issued after them
@@ -2118,76 +2212,474 @@ __global__ __launch_bounds__(quad_block) void k_quad(
   __builtin_amdgcn_s_setprio(0);
   __syncthreads();
 
-  // flush: the sum of the quadrant regions over the tile and its halo
-  const int RX = tile + hx;
-  const int RY = tile + hy;
-  const int R = RX * RY;
-  const int hs = halo_pixels(hx, hy);
-  double2* hk = halo + size_t(key) * hs;
-  // every pixel's value first, with the grid loads of the tile's pixels
-  // issued together, then the stores (at most quad_flush pixels per thread:
-  // the quadrant regions fit LDS only for hx, hy <= 13)
-  float4 val[quad_flush];
-  float2 g0[quad_flush];
-#pragma unroll
-  for (int j = 0; j < quad_flush; ++j) {
-    const int k = int(threadIdx.x) + j * quad_block;
-    val[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    g0[j] = make_float2(0.0f, 0.0f);
-    if (k >= R) continue;
-    const int ry = k / RX;
-    const int rx = k - ry * RX;
-    // (hi, lo) of the pixel: the quadrants' pairs combined by two-sum
-    float fr = 0.0f, fi = 0.0f, flr = 0.0f, fli = 0.0f;
-    bool first = true;
-#pragma unroll
-    for (int qd = 0; qd < 4; ++qd) {
-      const int ux = rx - (qd & 1) * quad_edge;
-      const int uy = ry - (qd >> 1) * quad_edge;
-      if (ux < 0 || uy < 0 || ux >= QX || uy >= QY) continue;
-      const float4 a = regions[qd * Q + uy * QX + ux];
-      if (first) {
-        fr = a.x;
-        fi = a.y;
-        flr = a.z;
-        fli = a.w;
-      } else {
-        float sum, err;
-        two_sum(fr, a.x, sum, err);
-        fr = sum;
-        flr = flr + (err + a.z);
-        two_sum(fi, a.y, sum, err);
-        fi = sum;
-        fli = fli + (err + a.w);
-      }
-      first = false;
-    }
-    val[j] = make_float4(fr, fi, flr, fli);
-    const int xx = x0 + rx, yy = y0 + ry;
-    if (rx < tile && ry < tile && xx < geo.nx && yy < geo.ny)
-      g0[j] = static_cast<const float2*>(grid)[(size_t(plane) * geo.ny + yy) * geo.nx + xx];
-  }
-#pragma unroll
-  for (int j = 0; j < quad_flush; ++j) {
-    const int k = int(threadIdx.x) + j * quad_block;
-    if (k >= R) continue;
-    const int ry = k / RX;
-    const int rx = k - ry * RX;
-    const float4 v = val[j];
-    if (rx < tile && ry < tile) {
-      const int xx = x0 + rx, yy = y0 + ry;
-      if (xx >= geo.nx || yy >= geo.ny) continue;
-      float sr, er, si, ei;
-      two_sum(g0[j].x, v.x, sr, er);
-      two_sum(g0[j].y, v.y, si, ei);
-      static_cast<float2*>(grid)[(size_t(plane) * geo.ny + yy) * geo.nx + xx] =
-        make_float2(sr + (er + v.z), si + (ei + v.w));
-    } else {
-      const int h = ry < tile ? ry * hx + (rx - tile) : tile * hx + (ry - tile) * RX + rx;
-      hk[h] = make_double2(double(v.x) + double(v.z), double(v.y) + double(v.w));
-    }
-  }
+  quad_flush_tile<quad_block, quad_flush>(regions, QX, QY, hx, hy, geo, plane, x0, y0,
+                                         static_cast<float2*>(grid),
+                                         halo + size_t(key) * halo_pixels(hx, hy));
   if (threadIdx.x == 0) halo_set[key] = 1u;
+}
+
+// ---- gridding: quadrant keys, two waves per quadrant, FP32 block sums ----
+//
+// k_sub grids FP32-compensated tiles for supports of at most 8 x 8 pixels
+// from quadrant-keyed bins (k_bin with quad_keys: quadrant q of tile key t
+// is bin key 4 t + q in every bin block).  The workgroup of a (plane, tile)
+// has eight waves; waves q and q + 4 own quadrant q (the entries whose
+// support corner lies in it) and take its entry list alternately, 64
+// entries at a time: lane j locates, loads and prepares entry j of the
+// batch (CF slices, region pixel, w-sign, flags).  The wave then adds the
+// batch's entries one at a time, one lane per CF tap, to its own FP32
+// region of (16 + hx) x (16 + hy) pixels in LDS by a plain read-modify-write
+// (an entry's taps are distinct pixels, the region is the wave's own and a
+// wave's LDS operations execute in order).  After every sub_fold entries
+// the wave folds the region into registers: lane l holds the (hi, lo)
+// pairs of region pixels l, l + 64, ..., adds each pixel's FP32 sum to its
+// pair by two-sum and clears the pixel.  So the products of a pixel are
+// summed in FP32 over at most sub_fold entries, and those sums are
+// accumulated with compensation.  The flush combines the pairs of a
+// quadrant's two waves, then the four quadrants over the tile and halo
+// (quad_flush_tile), as k_quad does, so k_halo folds the halos unchanged.
+constexpr int sub_waves = 8;
+constexpr int sub_block = sub_waves * wave;
+constexpr int sub_bins_max = 256;  // bin blocks per pass (LDS segment tables)
+constexpr int sub_fold = 32;       // entries between folds
+constexpr int sub_ring = 8;        // slots whose CF taps load ahead
+constexpr int sub_flush = 3;       // tile + halo pixels per thread in the flush
+
+// lane -> CF tap (X, Y) for supports up to 8 x 8: X = lane & 7, and the
+// 8-lane groups take rows 0, 4, 1, 5, 2, 6, 3, 7, so that each 16 lanes
+// read two rows 4 region rows apart
+__device__ __forceinline__ int sub_lane_y(int lane) {
+  const int g = lane >> 3;
+  return (g >> 1) + 4 * (g & 1);
+}
+
+// a batch entry prepared by its lane
+template <int N>
+struct SubPrep {
+  unsigned w0;        // region pixel | sx << 12 | sy << 18 | use << 24 | flags
+  unsigned slice[N];  // first tap of each polarization's CF slice
+  float2 g[N];
+};
+
+// exp(i phi) rounded to FP32 for an FP64 phase: phi is reduced to
+// [-pi, pi] in FP64 (2 pi split into a 53-bit head and its tail, exact for
+// the multiples used for |phi| < 2^20), then the FP32 sine and cosine of the
+// remainder.  Within 2 ulp of fl32(cos phi), fl32(sin phi) (the reference's
+// cphase in FP64, rounded) -- far inside the FP32-compensated grid's 1e-6.
+__device__ __forceinline__ float2 screen_f32(double phi) {
+  constexpr double two_pi_hi = 6.283185307179586;       // fl64(2 pi)
+  constexpr double two_pi_lo = 2.4492935982947064e-16;  // 2 pi - fl64(2 pi)
+  const double k = rint(phi * 0.15915494309189535);      // phi / (2 pi)
+  const double r = (phi - k * two_pi_hi) - k * two_pi_lo;
+  float sn, cs;
+  __sincosf(float(r), &sn, &cs);
+  return make_float2(cs, sn);
+}
+
+// an entry with a phase gradient: its CF taps times the phase screen
+// exp(i phi(X, Y)) (FP64 phase as the reference's, screen_f32), added by
+// one wave, one lane per tap, to the wave's FP32 region
+template <int N>
+__device__ __forceinline__ void sub_gradient_entry(
+  unsigned w0, const unsigned* slice, const float2* g, unsigned ri, unsigned pos,
+  const Geometry& geo, const float2* __restrict__ taps, const VisRec<N>* __restrict__ recs,
+  int QX, float2* region) {
+  const int lane = int(threadIdx.x % wave);
+  const int base = int(w0 & 4095u);
+  const int sx = int((w0 >> 12) & 63u);
+  const int sy = int((w0 >> 18) & 63u);
+  const unsigned use = (w0 >> 24) & 15u;
+  const float imf = (w0 & quad_pos_w) ? -1.0f : 1.0f;
+  const int fxi = int((pos >> 12) & 255u);
+  const int fyi = int((pos >> 20) & 255u);
+  const double gxr = recs[ri].grad[0];
+  const double gyr = recs[ri].grad[1];
+  const double phi0x = -gxr * ((sx / 2) * geo.os - (fxi - geo.F));
+  const double phi0y = -gyr * ((sy / 2) * geo.os - (fyi - geo.F));
+  const double dphix = gxr * geo.os;
+  const double dphiy = gyr * geo.os;
+  const int X = lane & 7, Y = sub_lane_y(lane);
+  if (X >= sx || Y >= sy) return;
+  const int tp = Y * sx + X;
+  const float2 scr = screen_f32((phi0x + X * dphix) + (phi0y + Y * dphiy));
+  double sr = 0.0, si = 0.0;
+  float h_r = 0.0f, h_i = 0.0f;
+  for (int v = 0; v < N; ++v) {
+    if (!((use >> v) & 1u)) continue;
+    float2 cc = taps[size_t(slice[v]) + size_t(tp)];
+    cc.y *= imf;
+    const float2 pr = cmul(cmul(cc, scr), g[v]);
+    if constexpr (N == 1) {
+      h_r = pr.x;
+      h_i = pr.y;
+    }
+    sr += double(pr.x);
+    si += double(pr.y);
+  }
+  if constexpr (N > 1) {
+    h_r = float(sr);
+    h_i = float(si);
+  }
+  float2* a = region + base + Y * QX + X;
+  float2 t = *a;
+  t.x += h_r;
+  t.y += h_i;
+  *a = t;
+}
+
+// PX: region pixels per lane; each wave's region is padded to PX * 64
+// pixels (>= (16 + hx) * (16 + hy)), so a lane folds pixels lane + 64 k with
+// no bounds test.  UNI: every CF group has the same support, whose lane mask
+// then holds for the whole call.
+template <int N, int PX, bool UNI>
+__global__ __launch_bounds__(sub_block) __attribute__((amdgpu_waves_per_eu(N == 1 ? 8 : 4)))
+void k_sub(
+  Geometry geo, const GroupInfo* __restrict__ groups, const float2* __restrict__ taps,
+  unsigned taps_bytes, Mueller mu, const unsigned* __restrict__ off_table, int nbins,
+  size_t block_cap,
+  const Entry4<N>* __restrict__ entries, const unsigned* __restrict__ entry_vis,
+  const VisRec<N>* __restrict__ recs, int hx, int hy, float2* __restrict__ grid,
+  double2* __restrict__ halo, unsigned* __restrict__ halo_set) {
+  // eight FP32 regions of QP = PX * 64 pixels (float2); in the flush the
+  // same bytes hold the four quadrants' (hi, lo) pairs (float4, Q each)
+  extern __shared__ float2 sregions[];
+  __shared__ unsigned seg_pref[4][sub_bins_max + 1];
+  __shared__ unsigned short seg_beg[4][sub_bins_max];
+  constexpr int QP = PX * wave;
+
+  const int QX = quad_edge + hx;
+  const int QY = quad_edge + hy;
+  const int Q = QX * QY;
+  const unsigned ntk = unsigned(geo.ntx) * geo.nty * geo.nr * geo.nc;
+  const unsigned tkey = xcd_key(blockIdx.x, gridDim.x);  // one block per tile key
+  const size_t row = size_t(4) * ntk + 1;                // offset-table row
+  const unsigned tid = threadIdx.x;
+  const int w = int(tid) / wave;
+  const int lane = int(tid) % wave;
+  const int q = w & 3;
+  const int half = w >> 2;
+
+  // segment tables of the four quadrant keys, then their exclusive scans
+  for (int i = int(tid); i < 4 * nbins; i += sub_block) {
+    const int bb = i >> 2, qq = i & 3;
+    const unsigned* tab = off_table + size_t(bb) * row + 4 * size_t(tkey) + qq;
+    const unsigned s0 = tab[0], s1 = tab[1];
+    seg_pref[qq][bb] = s1 - s0;
+    seg_beg[qq][bb] = (unsigned short)s0;
+  }
+  __syncthreads();
+  if (w < 4) {
+    unsigned* a = seg_pref[w];
+    unsigned v[4];
+    unsigned sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int idx = lane * 4 + k;
+      v[k] = idx < nbins ? a[idx] : 0u;
+      sum += v[k];
+    }
+    unsigned incl = sum;
+#pragma unroll
+    for (int d = 1; d < wave; d <<= 1) {
+      const unsigned t = __shfl_up(incl, d, wave);
+      if (lane >= d) incl += t;
+    }
+    unsigned run = incl - sum;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int idx = lane * 4 + k;
+      if (idx < nbins) a[idx] = run;
+      run += v[k];
+    }
+    if (lane == wave - 1) a[nbins] = incl;
+  }
+  __syncthreads();
+  if (seg_pref[0][nbins] + seg_pref[1][nbins] + seg_pref[2][nbins] + seg_pref[3][nbins] == 0u) {
+    if (tid == 0) halo_set[tkey] = 0u;
+    return;
+  }
+
+  const unsigned ntiles = unsigned(geo.ntx) * geo.nty;
+  const unsigned plane = tkey / ntiles;
+  const int t = int(tkey % ntiles);
+  const int x0 = (t % geo.ntx) * tile;
+  const int y0 = (t / geo.ntx) * tile;
+  const int gpol = int(plane % geo.nr);
+  const int qx = q & 1, qy = q >> 1;
+
+  float2* region = sregions + size_t(w) * QP;
+#pragma unroll
+  for (int k = 0; k < PX; ++k) region[lane + k * wave] = make_float2(0.0f, 0.0f);
+  float4 acc[PX];
+#pragma unroll
+  for (int k = 0; k < PX; ++k) acc[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  // the FP32 sums of the region into the (hi, lo) pairs, by two-sum
+  auto fold = [&]() {
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+      const int px = lane + k * wave;
+      const float2 p = region[px];
+      region[px] = make_float2(0.0f, 0.0f);
+      float sum, err;
+      two_sum(acc[k].x, p.x, sum, err);
+      acc[k].x = sum;
+      acc[k].z += err;
+      two_sum(acc[k].y, p.y, sum, err);
+      acc[k].y = sum;
+      acc[k].w += err;
+    }
+  };
+
+  const bool one_group = geo.ngroups == 1;
+  const GroupInfo group0 = groups[0];
+  // this plane's Mueller rows for both signs of w
+  int row_m[N], row_mc[N];
+#pragma unroll
+  for (int v = 0; v < N; ++v) {
+    row_m[v] = mu.M[gpol][v];
+    row_mc[v] = mu.Mc[gpol][v];
+  }
+  const unsigned total = seg_pref[q][nbins];
+  auto locate = [&](unsigned i) {
+    int lo = 0, hi = nbins;  // seg_pref[q][lo] <= i < seg_pref[q][hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (seg_pref[q][mid] <= i) lo = mid; else hi = mid;
+    }
+    return unsigned(lo) * unsigned(block_cap) + seg_beg[q][lo] + (i - seg_pref[q][lo]);
+  };
+  const int lane_x = lane & 7;
+  const int lane_y = sub_lane_y(lane);
+  // LDS byte offset of this lane's tap pixel in the wave's region, relative
+  // to an entry's support corner
+  const unsigned lane_addr =
+    unsigned((size_t(w) * QP + size_t(lane_y * QX + lane_x)) * sizeof(float2));
+  // CF taps through a buffer resource: the slice's byte offset goes in a
+  // scalar register and the lane's tap offset in a 32-bit vector register;
+  // reads past taps_bytes return 0
+  const __amdgpu_buffer_rsrc_t tap_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+    const_cast<float2*>(taps), 0, int(taps_bytes), 0x00020000);
+  // lanes holding a tap of a support (sx, sy), and their tap byte offset
+  auto support_lanes = [&](int sx, int sy, unsigned& off) {
+    const bool ok = lane_x < sx && lane_y < sy;
+    off = unsigned(ok ? lane_y * sx + lane_x : 0) * unsigned(sizeof(float2));
+    return ok;
+  };
+  unsigned tap_off = 0;
+  const bool uni_ok = UNI ? support_lanes(group0.sx, group0.sy, tap_off) : true;
+  int tap_key = -1;
+  unsigned long long tap_lanes = 0ull;
+
+  for (unsigned b0 = unsigned(half) * wave; b0 < total; b0 += 2 * wave) {
+    // lane j: entry b0 + j of the quadrant's list, prepared.  Slots that
+    // add nothing here (past the list, phase gradient, no CF index) keep
+    // zero values and the region's first pixel, and read a valid CF slice.
+    const unsigned i = b0 + unsigned(lane);
+    const bool valid = i < total;
+    unsigned ei = 0;
+    Entry4<N> e{};
+    if (valid) {
+      ei = locate(i);
+      e = entries[ei];
+    }
+    // p_addr: the entry's LDS byte offset from the region origin, with the
+    // w-sign flip (bit 31: conj(c) for w > 0) -- 0 for slots that add nothing
+    unsigned p_addr = 0, p_key = 0;
+    unsigned p_sl[N];
+    float p_gx[N], p_gy[N];
+#pragma unroll
+    for (int v = 0; v < N; ++v) {
+      p_sl[v] = 0u;
+      p_gx[v] = p_gy[v] = 0.0f;
+    }
+    bool grad = false, normal = false;
+    if (valid) {
+      const unsigned pos = e.h.pos;
+      const int lx = int(pos & 63u);
+      const int ly = int((pos >> 6) & 63u);
+      const int fxi = int((pos >> 12) & 255u);
+      const int fyi = int((pos >> 20) & 255u);
+      const bool pos_w = pos & e4_pos_w;
+      grad = pos & e4_gradient;
+      const int grp = int(e.h.gpk & 0xffffu);
+      const int cfcube = int(e.h.gpk >> 16);
+      const GroupInfo g = one_group ? group0 : groups[grp];
+      const long long slice_taps = (long long)g.sx * g.sy;
+      unsigned use = 0;
+#pragma unroll
+      for (int v = 0; v < N; ++v) {
+        const int m = pos_w ? row_m[v] : row_mc[v];
+        const bool u = m >= 0 && m < g.nmueller;
+        use |= u ? 1u << v : 0u;
+        p_sl[v] = unsigned(
+          g.tap_offset
+          + (((long long)(cfcube * g.nmueller + (u ? m : 0)) * geo.nf + fxi) * geo.nf + fyi)
+            * slice_taps);
+        if (u && !grad) {
+          p_gx[v] = e.g[v].x;
+          p_gy[v] = e.g[v].y;
+        }
+      }
+      const unsigned base = unsigned((ly - qy * quad_edge) * QX + (lx - qx * quad_edge));
+      p_key = unsigned(g.sx) | (unsigned(g.sy) << 6);
+      normal = !grad && use != 0u;
+      if (normal) p_addr = base * unsigned(sizeof(float2)) | (pos_w ? 0x80000000u : 0u);
+      grad = grad && use != 0u;
+    }
+    const unsigned long long nmask = __ballot(normal);
+    const unsigned long long gtodo = __ballot(valid && grad);
+    // slots up to the last entry added here
+    const int nslots = nmask ? 64 - __builtin_clzll(nmask) : 0;
+
+    // CF taps of slot j: loaded sub_ring slots ahead into ring[j % sub_ring]
+    // (the fetches run on the whole wave: a lane's prepared values are read
+    // from it while it is active); the slot's other values are read from
+    // its lane when it is added
+    float2 ring[sub_ring][N];
+    unsigned long long ring_lanes[sub_ring];  // !UNI: lanes of the slot's support
+    auto fetch = [&](int j) {
+      const int r = j % sub_ring;
+      if constexpr (!UNI) {
+        const int key_s = __builtin_amdgcn_readlane(int(p_key), j);
+        if (key_s != tap_key) {
+          tap_key = key_s;
+          tap_lanes = __ballot(support_lanes(key_s & 63, key_s >> 6, tap_off));
+        }
+        ring_lanes[r] = tap_lanes;
+      }
+#pragma unroll
+      for (int v = 0; v < N; ++v) {
+        const int slv = __builtin_amdgcn_readlane(int(p_sl[v]), j);
+        const v2i_t raw = __builtin_amdgcn_raw_buffer_load_b64(
+          tap_rsrc, int(tap_off), slv * int(sizeof(float2)), 0);
+        ring[r][v] = make_float2(__int_as_float(raw.x), __int_as_float(raw.y));
+      }
+    };
+    auto add = [&](int j) {
+      const int r = j % sub_ring;
+      const unsigned aw = unsigned(__builtin_amdgcn_readlane(int(p_addr), j));
+      const unsigned flip = aw & 0x80000000u;
+      const unsigned addr = aw & 0x7fffffffu;
+      bool on;
+      if constexpr (UNI) on = uni_ok; else on = __builtin_amdgcn_inverse_ballot_w64(ring_lanes[r]);
+      float h_r = 0.0f, h_i = 0.0f;
+      double sr = 0.0, si = 0.0;
+#pragma unroll
+      for (int v = 0; v < N; ++v) {
+        const unsigned ux = unsigned(__builtin_amdgcn_readlane(__float_as_int(p_gx[v]), j));
+        const unsigned uy = unsigned(__builtin_amdgcn_readlane(__float_as_int(p_gy[v]), j));
+        const float gx = __uint_as_float(ux), gy = __uint_as_float(uy);
+        // conj(c) g for w > 0: c.y g.y and c.y g.x enter with the sign flipped
+        const float gcx = __uint_as_float(uy ^ flip), gcy = __uint_as_float(ux ^ flip);
+        // each product rounded as the reference's FP32 complex product
+        const float2 c = ring[r][v];
+        const float pr = c.x * gx - c.y * gcx;
+        const float pi = c.x * gy + c.y * gcy;
+        if constexpr (N == 1) {
+          h_r = pr;
+          h_i = pi;
+        } else {
+          sr += double(pr);
+          si += double(pi);
+        }
+      }
+      if constexpr (N > 1) {
+        h_r = float(sr);
+        h_i = float(si);
+      }
+      if (on) {
+        float2* a =
+          reinterpret_cast<float2*>(reinterpret_cast<char*>(sregions) + addr + lane_addr);
+        float2 tv = *a;
+        tv.x += h_r;
+        tv.y += h_i;
+        *a = tv;
+      }
+    };
+    // two halves of 32 slots, each followed by a fold (so at most 32
+    // entries between folds)
+#pragma unroll
+    for (int hb = 0; hb < 2; ++hb) {
+      if (hb * 32 < nslots) {
+#pragma unroll
+        for (int j = 0; j < sub_ring; ++j) fetch(hb * 32 + j);
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+          add(hb * 32 + j);
+          if (j + sub_ring < 32) fetch(hb * 32 + j + sub_ring);
+        }
+        fold();
+      }
+    }
+    // entries with a phase gradient, after the batch's others
+    unsigned long long gl = gtodo;
+    if (gl) {
+      int since = 0;
+      while (gl) {
+        const int j = __builtin_ctzll(gl);
+        gl &= gl - 1ull;
+        // the entry again (a scalar load), so that the batch's raw entries
+        // need not stay in registers through the loop above
+        const unsigned ej = unsigned(__builtin_amdgcn_readlane(int(ei), j));
+        const Entry4<N> ge = entries[ej];
+        const unsigned gpos = ge.h.pos;
+        const bool gpos_w = gpos & e4_pos_w;
+        const GroupInfo gg = one_group ? group0 : groups[ge.h.gpk & 0xffffu];
+        unsigned guse = 0;
+        unsigned sl[N];
+        float2 gv[N];
+#pragma unroll
+        for (int v = 0; v < N; ++v) {
+          const int m = gpos_w ? row_m[v] : row_mc[v];
+          guse |= (m >= 0 && m < gg.nmueller) ? 1u << v : 0u;
+          sl[v] = unsigned(__builtin_amdgcn_readlane(int(p_sl[v]), j));
+          gv[v] = ge.g[v];
+        }
+        const unsigned gbase = ((gpos >> 6) & 63u) - unsigned(qy * quad_edge);
+        const unsigned w0 = (gbase * unsigned(QX) + ((gpos & 63u) - unsigned(qx * quad_edge)))
+          | (unsigned(gg.sx) << 12) | (unsigned(gg.sy) << 18) | (guse << 24)
+          | (gpos_w ? quad_pos_w : 0u);
+        sub_gradient_entry<N>(w0, sl, gv, entry_vis[ej], ge.h.pos, geo, taps, recs, QX, region);
+        if (++since >= sub_fold) {
+          fold();
+          since = 0;
+        }
+      }
+      fold();
+    }
+  }
+  __syncthreads();
+
+  // combine the pairs of the quadrant's two waves, then flush
+  float4* pairs = reinterpret_cast<float4*>(sregions);
+  if (half == 1) {
+#pragma unroll
+    for (int k = 0; k < PX; ++k)
+      if (lane + k * wave < Q) pairs[q * Q + lane + k * wave] = acc[k];
+  }
+  __syncthreads();
+  if (half == 0) {
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+      const int px = lane + k * wave;
+      if (px < Q) {
+        const float4 b = pairs[q * Q + px];
+        float sum, err;
+        float4 r;
+        two_sum(acc[k].x, b.x, sum, err);
+        r.x = sum;
+        r.z = acc[k].z + (err + b.z);
+        two_sum(acc[k].y, b.y, sum, err);
+        r.y = sum;
+        r.w = acc[k].w + (err + b.w);
+        pairs[q * Q + px] = r;
+      }
+    }
+  }
+  __syncthreads();
+  quad_flush_tile<sub_block, sub_flush>(pairs, QX, QY, hx, hy, geo, plane, x0, y0, grid,
+                                        halo + size_t(tkey) * halo_pixels(hx, hy));
+  if (tid == 0) halo_set[tkey] = 1u;
 }
 
 // ---- degridding --------------------------------------------------------
@@ -2930,9 +3422,10 @@ hipError_t launch_bin(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool update_weights, bool gvis_ready, GvisSource src,
   const double2* residuals, CallBuffers& cb, size_t block_cap, double* weights,
-  bool halo_entries, bool degrid_pass, void* copy_records) {
+  bool halo_entries, bool degrid_pass, void* copy_records, bool quad_keys) {
   if (n == 0) return hipSuccess;
   if (copy_records && !degrid_pass) return hipErrorInvalidValue;
+  if (quad_keys && (!halo_entries || degrid_pass)) return hipErrorInvalidValue;
   const int adj = src == GvisSource::Adjoint;
   const int dg = degrid_pass;
   const unsigned nb = unsigned(bin_blocks_for(n));
@@ -2942,7 +3435,7 @@ hipError_t launch_bin(
     static_cast<const VisRec<NP>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,    \
     int(update_weights), int(gvis_ready), adj, dg, cb.gvis, residuals,         \
     cb.off_table, cb.stream_entries, cb.entry_vis, block_cap, weights,         \
-    cb.invalid_count, static_cast<VisRec<NP>*>(copy_records))
+    cb.invalid_count, static_cast<VisRec<NP>*>(copy_records), int(quad_keys))
   if (halo_entries) {
     switch (npol) {
     case 1: HPGX_BIN(1, true); break;
@@ -3010,6 +3503,59 @@ int quad_bins_limit() { return quad_bins_max; }
 bool quad_supported(int hx, int hy, int npol) {
   return hx >= 0 && hy >= 0 && hx < tile && hy < tile && quad_lds_bytes(hx, hy, npol) <= 65536
     && (tile + hx) * (tile + hy) <= quad_flush * quad_block;
+}
+
+size_t sub_lds_bytes(int hx, int hy) {
+  const int px = ((quad_edge + hx) * (quad_edge + hy) + wave - 1) / wave;
+  const int pxc = px <= 4 ? 4 : px <= 6 ? 6 : px <= 8 ? 8 : 9;
+  return size_t(sub_waves) * pxc * wave * sizeof(float2);
+}
+int sub_bins_limit() { return sub_bins_max; }
+bool sub_supported(int hx, int hy) { return hx >= 0 && hy >= 0 && hx < 8 && hy < 8; }
+
+hipError_t launch_tile_sub(
+  hipStream_t s, int npol, size_t n, size_t ntile_keys, const Geometry& geo, const DeviceCF& cf,
+  size_t taps_bytes, bool uni, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
+  int hx, int hy, void* grid) {
+  if (ntile_keys == 0 || n == 0) return hipSuccess;
+  const int nb = bin_blocks_for(n);
+  // segment starts are 16-bit offsets within a bin block; CF taps are
+  // addressed by 32-bit byte offsets
+  if (nb > sub_bins_max || !sub_supported(hx, hy) || block_cap > 65536
+      || taps_bytes > 0xffffffffull)
+    return hipErrorInvalidValue;
+  const int Q = (quad_edge + hx) * (quad_edge + hy);
+  const int px = (Q + wave - 1) / wave;  // 4 .. 9
+  const int pxc = px <= 4 ? 4 : px <= 6 ? 6 : px <= 8 ? 8 : 9;
+  const size_t dyn = size_t(sub_waves) * pxc * wave * sizeof(float2);
+#define HPGX_SUB(NP, PX)                                                       \
+  hipLaunchKernelGGL(                                                          \
+    uni ? (k_sub<NP, PX, true>) : (k_sub<NP, PX, false>),                      \
+    dim3(unsigned(ntile_keys)), dim3(sub_block), dyn, s,                       \
+    geo, cf.groups, cf.taps, unsigned(taps_bytes), mu, cb.off_table, nb,       \
+    block_cap,                                                                 \
+    static_cast<const Entry4<NP>*>(cb.stream_entries), cb.entry_vis,           \
+    static_cast<const VisRec<NP>*>(cb.vis), hx, hy, static_cast<float2*>(grid),\
+    cb.halo, cb.halo_set)
+#define HPGX_SUB_PX(NP)                                                        \
+  switch (pxc) {                                                               \
+  case 4: HPGX_SUB(NP, 4); break;                                              \
+  case 6: HPGX_SUB(NP, 6); break;                                              \
+  case 8: HPGX_SUB(NP, 8); break;                                              \
+  default: HPGX_SUB(NP, 9); break;                                             \
+  }
+  switch (npol) {
+  case 1: HPGX_SUB_PX(1); break;
+  case 2: HPGX_SUB_PX(2); break;
+  case 3: HPGX_SUB_PX(3); break;
+  case 4: HPGX_SUB_PX(4); break;
+  default: return hipErrorInvalidValue;
+  }
+#undef HPGX_SUB_PX
+#undef HPGX_SUB
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_halo(s, 1, ntile_keys, geo, cb, hx, hy, grid);
 }
 
 hipError_t launch_tile_halo(
