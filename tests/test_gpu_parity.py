@@ -138,14 +138,93 @@ def test_random_vs_oracle_fp32_compensated(hpg, oracle_mod, case):
 @pytest.mark.parametrize("gridder", [2, 3, 4, 5])
 @pytest.mark.parametrize("case", [0, 2, 3, 4])
 @pytest.mark.parametrize("mode", [0, 1])
+def test_grid_without_weights_vs_oracle(hpg, oracle_mod, monkeypatch, gridder, case, mode):
+    """grid_vis_no_weights (src/hpg_core.hpp:995-1068, 1221-1251): gridding
+    with update_grid_weights false adds to the grid exactly as the oracle
+    does and leaves the weights bit for bit as they were (here: nonzero,
+    from a first call with weights).  FP64 mode 1e-12 of max|grid| per
+    plane, FP32-compensated e_max, e_rms <= 1e-6."""
+    from hpg_amd import workloads
+    monkeypatch.setenv("HPGX_GRIDDER", str(gridder))
+    npol, gsz, M, Mc, radii, nm, nc, os_ = CASES[case]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=140 + case)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (0.9, 1.1), M, cf, mode, Mc)
+    first = workloads.random_visibilities(800, npol, gsz, (0.9, 1.1), cf, seed=150 + case,
+                                          phase=True, margin=-3)
+    second = workloads.random_visibilities(1200, npol, gsz, (0.9, 1.1), cf, seed=160 + case,
+                                           phase=True, margin=-3)
+    print(f"no weights: gridder {gridder} case {case} mode {mode}", flush=True)
+    gs.grid_visibilities(hpg.Device.Serial, first, True)
+    o.grid_visibilities(oracle_view(first, oracle_mod), True)
+    before = gs.grid_weights()
+    assert np.abs(before).max() > 0
+    gs.grid_visibilities(hpg.Device.Serial, second, False)
+    o.grid_visibilities(oracle_view(second, oracle_mod), False)
+    assert np.array_equal(gs.grid_weights(), before)
+    np.testing.assert_allclose(before, o.grid_weights(), rtol=1e-12)
+    got, exp = gs.grid_values(), o.grid_values()
+    tol = 1e-12 if mode == 0 else 1e-6
+    for c in range(gsz[3]):
+        for r in range(gsz[2]):
+            if np.abs(exp[c, r]).max() == 0:
+                assert np.abs(got[c, r]).max() == 0
+                continue
+            e_max, e_rms = rel_errors(got[c, r], exp[c, r])
+            assert e_max <= tol and e_rms <= max(tol, 1e-12), (c, r, e_max, e_rms)
+
+
+QUAD_CASES = [
+    # npol, grid_size, M, Mc, radii, nmueller, ncube, os: supports of at most
+    # 8 pixels, one support for all groups or several
+    (1, (40, 36, 1, 1), [[0]], [[0]], [3], 1, 1, 10),
+    (2, (96, 80, 2, 2), [[0, 1], [2, 3]], [[1, 0], [3, 2]], [1, (3, 2), 2], 4, 2, 6),
+    (4, (70, 66, 4, 1), [[0, 1, 2, 3], [4, 5, 6, 7], [8, 9, 10, 11], [12, 13, 14, 15]],
+     [[0, 1, 2, 3], [4, 5, 6, 7], [8, 9, 10, 11], [12, 13, 14, 15]], [3, 1], 16, 1, 5),
+    (3, (50, 50, 2, 1), [[0, -1, 1], [2, 0, -1]], [[1, -1, 0], [-1, 2, 0]], [3, 3], 3, 1, 7),
+]
+
+
+@pytest.mark.parametrize("sub", ["1", "0"])
+@pytest.mark.parametrize("case", range(len(QUAD_CASES)))
+def test_quadrant_kernels_vs_oracle(hpg, oracle_mod, monkeypatch, sub, case):
+    """FP32-compensated grids with supports up to 8 pixels through the
+    quadrant-key kernel (k_sub, HPGX_SUB_TILES=1, the default) and through
+    the round-based quadrant kernel (k_quad, HPGX_SUB_TILES=0), with phase
+    gradients on a third of the records: e_max, e_rms <= 1e-6 per plane."""
+    from hpg_amd import workloads
+    monkeypatch.setenv("HPGX_SUB_TILES", sub)
+    npol, gsz, M, Mc, radii, nm, nc, os_ = QUAD_CASES[case]
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=170 + case)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (0.9, 1.1), M, cf, 1, Mc)
+    rec = workloads.random_visibilities(2500, npol, gsz, (0.9, 1.1), cf, seed=180 + case,
+                                        phase=True, margin=-3, gradient=True)
+    rec["cf_phase_gradient"][1::3] = 0.0
+    rec["cf_phase_gradient"][2::3] = 0.0
+    gs.grid_visibilities(hpg.Device.Serial, rec, True)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    got, exp = gs.grid_values(), o.grid_values()
+    for c in range(gsz[3]):
+        for r in range(gsz[2]):
+            if np.abs(exp[c, r]).max() == 0:
+                assert np.abs(got[c, r]).max() == 0
+                continue
+            e_max, e_rms = rel_errors(got[c, r], exp[c, r])
+            assert e_max <= 1e-6 and e_rms <= 1e-6, (c, r, e_max, e_rms)
+    np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
+
+
+@pytest.mark.parametrize("gridder", [2, 3, 4, 5])
+@pytest.mark.parametrize("case", [0, 2, 3, 4])
+@pytest.mark.parametrize("mode", [0, 1])
 def test_each_gridder_vs_oracle(hpg, oracle_mod, monkeypatch, gridder, case, mode):
     """Every gridding pipeline (HPGX_GRIDDER, read when a state is created:
     2 globally binned entries, 3 block-local bins with
     clipped tiles, 4 block-local bins with haloed tiles updated by LDS
     atomics -- which falls back to 3 for case 3, whose 35-pixel support
     exceeds a tile -- and 5 the haloed tiles added per quadrant wave, one
-    lane per tap, falling back to 4 where the quadrant regions do not fit
-    LDS) against the oracle, with phase gradients on a third of the records and some records
+    lane per tap (FP32-compensated grids: the quadrant-key kernel for
+    supports up to 8 pixels, the round-based one up to 14), falling back to
+    4 where the quadrant regions do not fit LDS) against the oracle, with phase gradients on a third of the records and some records
     off the grid.  FP64 mode:
     1e-6 of max|grid| (gradient screens); FP32-compensated: e_max, e_rms
     <= 1e-6 per plane."""
