@@ -136,6 +136,191 @@ def run_degrid(hpg, dev_records, cf, model, steps, warmup):
     return ms
 
 
+def run_host_batches(hpg, records, cf, tasks, calls):
+    """End-to-end gridding from host memory (the minigridder's timed loop,
+    src/minigridder.cc:753-782): `calls` grid_visibilities calls back to back
+    on the host batch, the upload of each call's records included, on a
+    state with `tasks` added tasks (tasks > 0: each call's upload overlaps
+    the previous calls' kernels).  Returns wall ms per call."""
+    import torch
+    gs = hpg.GridderState.create(hpg.Device.Hip, tasks, NUM_VIS, cf, (GRID, GRID, 1, 1),
+                                 (1.0, 1.0), [[0]], grid_mode=hpg.GridMode.FP32_COMPENSATED)
+    gs.set_convolution_function(hpg.Device.Serial, cf)
+    for _ in range(2):
+        gs.grid_visibilities(hpg.Device.Serial, records)
+    gs.fence()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        gs.grid_visibilities(hpg.Device.Serial, records)
+    gs.fence()
+    wall = (time.perf_counter() - t0) / calls
+    gs.release()
+    return wall * 1e3
+
+
+# ---- the multi-polarization workloads, one timed call each -------------------
+
+def _inside_support_sq(torch, rec_fields, radii, grid_size):
+    """per record: S^2 of its CF group if its support lies inside the grid
+    (grid_scale 1, frequency c: position u + N/2), else 0"""
+    grp = rec_fields["cf_index"][:, 1].long()
+    r = torch.as_tensor(radii, device=grp.device)[grp]
+    s = 2 * r + 1
+    ok = torch.ones_like(grp, dtype=torch.bool)
+    for d in range(2):
+        pos = rec_fields["uvw"][:, d] + grid_size[d] / 2.0
+        corner = torch.round(pos).long() - r
+        ok &= (corner >= 0) & (corner + s <= grid_size[d])
+    return torch.where(ok, s * s, torch.zeros_like(s)).double()
+
+
+def _cf_bytes(radii, os_, nmueller):
+    return sum((2 * r + 1 + 4) ** 2 * os_ ** 2 * nmueller * 8 for r in radii)
+
+
+def _time_call(gs, fn):
+    import torch
+    torch.cuda.synchronize()
+    gs.fence()
+    gs.event_record(8)
+    fn()
+    gs.event_record(9)
+    gs.fence()
+    return gs.event_elapsed_ms(8, 9) * 1e-3
+
+
+def config_legs(hpg):
+    """Device time, vis/s, FP32-issue and HBM fractions of the full-pol
+    W-projection (4096^2 x 4 x 8, 64 groups, 5e7 VisData<4>), differentiable
+    (2048^2 x 4, 16 groups, 2e7) and A/W-projection (8192^2 x 4 x 2, 512
+    groups, 2.5e7) workloads, one timed call each after a warm-up call on
+    the first 1e6 records.  FLOPs: 8 per (tap, vpol) gridded and 10
+    degridded, over records whose support lies inside the grid; bytes: the
+    records, the CF in the reference's layout, the grid read and written,
+    the weights, the model read when degridding."""
+    import torch
+    from hpg_amd import workloads
+    out = {}
+    M = workloads.MUELLER_FULL
+
+    def fractions(flops, nbytes, sec):
+        return {"fp32_frac": flops / sec / (FP32_PEAK_TFLOPS * 1e12),
+                "hbm_frac": nbytes / sec / (HBM_PEAK_GBS * 1e9),
+                "tflops": flops / sec / 1e12}
+
+    # full-pol W-projection
+    gsz = (4096, 4096, 4, 8)
+    radii, chirps = workloads.config2_geometry()
+    n = 50_000_000
+    cf = workloads.taper_chirp_cf_device(20, radii, chirps, 16)
+    rec = workloads.gaussian_uv_device_records(
+        n, 4, gsz, lambda w: torch.clamp((w * 64).floor(), max=63), 0.2, ncube=8)
+    fields = _record_fields(torch, rec, 4)
+    s2 = float(_inside_support_sq(torch, fields, radii, gsz).sum())
+    gs = hpg.GridderState.create(hpg.Device.Hip, 0, n, cf, gsz, (1.0, 1.0), M, M,
+                                 grid_mode=hpg.GridMode.FP64)
+    gs.set_convolution_function(hpg.Device.Serial, cf)
+    del cf
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    shape = (8, 4, 4096, 4096)
+    gs.set_model(hpg.Device.Serial, torch.complex(
+        torch.randn(shape, generator=g, device="cuda", dtype=torch.float64),
+        torch.randn(shape, generator=g, device="cuda", dtype=torch.float64)))
+    gs.degrid_grid_visibilities(hpg.Device.Serial, rec[:1_000_000])
+    gs.reset_grid()
+    sec = _time_call(gs, lambda: gs.degrid_grid_visibilities(hpg.Device.Serial, rec))
+    grid_b = 4096 * 4096 * 4 * 8 * 16
+    flops = s2 * 16 * (8 + 10)
+    nbytes = n * 120 + _cf_bytes(radii, 20, 16) + 3 * grid_b + 2 * 8 * 32
+    out["fullpol_wproj_4096sq_5e7"] = dict(
+        call="degrid_grid_visibilities (FP64 grid, model set)", records=n,
+        device_s=sec, vis_per_s=4 * n / sec, **fractions(flops, nbytes, sec))
+    gs.release()
+    del rec, fields
+    torch.cuda.empty_cache()
+
+    # differentiable degridding: forward and adjoint
+    gsz = (2048, 2048, 4, 1)
+    radii = [4 + (12 * k) // 15 for k in range(16)]
+    chirps = [0.002 * k for k in range(16)]
+    n = 20_000_000
+    Md = [[0, -1, -1, -1], [-1, 1, -1, -1], [-1, -1, 2, -1], [-1, -1, -1, 3]]
+    cf = workloads.taper_chirp_cf_device(16, radii, chirps, 4)
+    rec = workloads.gaussian_uv_device_records(
+        n, 4, gsz, lambda w: torch.clamp((w * 16).floor(), max=15), 0.15, seed=61)
+    s2 = float(_inside_support_sq(torch, _record_fields(torch, rec, 4), radii, gsz).sum())
+    gs = hpg.GridderState.create(hpg.Device.Hip, 0, n, cf, gsz, (1.0, 1.0), Md, Md)
+    gs.set_convolution_function(hpg.Device.Serial, cf)
+    del cf
+    shape = (1, 4, 2048, 2048)
+    gs.set_model(hpg.Device.Serial, torch.complex(
+        torch.randn(shape, generator=g, device="cuda", dtype=torch.float64),
+        torch.randn(shape, generator=g, device="cuda", dtype=torch.float64)))
+    gs.degrid_forward(rec[:1_000_000])
+    fwd = _time_call(gs, lambda: gs.degrid_forward(rec))
+    resid = torch.randn((n, 4), dtype=torch.complex128, device="cuda")
+    gs.degrid_adjoint(rec[:1_000_000], resid[:1_000_000])
+    adj = _time_call(gs, lambda: gs.degrid_adjoint(rec, resid))
+    grid_b = 2048 * 2048 * 4 * 16
+    out["differentiable_2048sq_2e7"] = dict(
+        forward=dict(call="degrid_forward (FP64 predictions)", records=n, device_s=fwd,
+                     vis_per_s=4 * n / fwd,
+                     **fractions(s2 * 4 * 10, n * (120 + 64) + _cf_bytes(radii, 16, 4)
+                                 + grid_b, fwd)),
+        adjoint=dict(call="degrid_adjoint (FP64 grid)", records=n, device_s=adj,
+                     vis_per_s=4 * n / adj,
+                     **fractions(s2 * 4 * 8, n * (120 + 64) + _cf_bytes(radii, 16, 4)
+                                 + 2 * grid_b, adj)))
+    gs.release()
+    del rec, resid
+    torch.cuda.empty_cache()
+
+    # A/W-projection, one GPU's share (2 of 16 channels)
+    gsz = (8192, 8192, 4, 2)
+    radii, chirps = workloads.config5_geometry()
+    n = 25_000_000
+    cf = workloads.taper_chirp_cf_device(10, radii, chirps, 16, phase_screen=True)
+    pa = torch.randint(0, 16, (n,), generator=g, device="cuda")
+    rec = workloads.gaussian_uv_device_records(
+        n, 4, gsz, lambda w: torch.clamp((w * 32).floor(), max=31).long() * 16 + pa, 0.25,
+        weight_range=(1e-6, 1.0), ncube=2, seed=52)
+    s2 = float(_inside_support_sq(torch, _record_fields(torch, rec, 4), radii, gsz).sum())
+    gs = hpg.GridderState.create(hpg.Device.Hip, 0, n, cf, gsz, (1.0, 1.0), M, M,
+                                 grid_mode=hpg.GridMode.FP32_COMPENSATED)
+    gs.set_convolution_function(hpg.Device.Serial, cf)
+    del cf
+    torch.cuda.empty_cache()
+    shape = (2, 4, 8192, 8192)
+    gs.set_model(hpg.Device.Serial, torch.complex(
+        torch.randn(shape, generator=g, device="cuda", dtype=torch.float64),
+        torch.randn(shape, generator=g, device="cuda", dtype=torch.float64)))
+    torch.cuda.empty_cache()
+    gs.degrid_grid_visibilities(hpg.Device.Serial, rec[:1_000_000])
+    gs.reset_grid()
+    sec = _time_call(gs, lambda: gs.degrid_grid_visibilities(hpg.Device.Serial, rec))
+    grid_b = 8192 * 8192 * 4 * 2 * 8
+    flops = s2 * 16 * (8 + 10)
+    nbytes = n * 120 + _cf_bytes(radii, 10, 16) + 2 * grid_b + 8192 * 8192 * 8 * 16
+    out["awproj_8192sq_2ch_2.5e7"] = dict(
+        call="degrid_grid_visibilities (FP32-compensated grid, FP64 model)", records=n,
+        device_s=sec, vis_per_s=4 * n / sec, **fractions(flops, nbytes, sec))
+    gs.release()
+    del rec
+    torch.cuda.empty_cache()
+    return out
+
+
+def _record_fields(torch, rec, npol):
+    """typed views of the cf_index and uvw fields of device records"""
+    dt = __import__("hpg_amd").visdata_dtype(npol)
+    oc = dt.fields["cf_index"][1]
+    ou = dt.fields["uvw"][1]
+    return {"cf_index": rec[:, oc:oc + 8].contiguous().view(torch.int32),
+            "uvw": rec[:, ou:ou + 24].contiguous().view(torch.float64)}
+
+
 def degrid_bytes(num_vis):
     """Compulsory HBM bytes of one degridding call: the records read and
     written back whole (with their predicted values), the CF and the FP64
@@ -327,6 +512,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline and precision legs")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the multi-polarization workloads' legs")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run the program profiled by the counter runs and exit")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -372,6 +559,8 @@ def main():
         else:
             os.environ["HPGX_FP32_TILE"] = prev
 
+    e2e = {f"tasks_{t}": run_host_batches(hpg, records, cf, t, 10) for t in (0, 2)}
+
     rng = np.random.default_rng(7)
     model = (rng.standard_normal((1, 1, GRID, GRID))
              + 1j * rng.standard_normal((1, 1, GRID, GRID)))
@@ -404,7 +593,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_per_call("fp32c"),
                      "copy_kernel_gbs": copy_gbs,
-                     "kernel": "grid_visibilities call (k_bin + k_quad + k_halo)",
+                     "kernel": "grid_visibilities call (k_bin + k_sub + k_halo)",
                      "bytes_per_call": nbytes, "device_ms_per_call": dev_ms,
                      "fp32_frac": flops / (dev_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS},
         "finishing_ms": fin,
@@ -424,8 +613,15 @@ def main():
                    "bytes_per_call": degrid_bytes(NUM_VIS),
                    "hbm_frac": degrid_bytes(NUM_VIS) / (degrid_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                    "kernel": "k_bin (also writes the returned batch) + k_degrid4"},
+        "end_to_end": {
+            "what": "grid_visibilities from host memory, records uploaded each call "
+                    "(10 calls back to back), FP32-compensated grid",
+            **{k: {"ms_per_call": v, "vis_per_s": NUM_VIS / (v * 1e-3)}
+               for k, v in e2e.items()}},
         "cpu_baseline": None,
     }
+    if not args.no_configs:
+        out["configs"] = config_legs(hpg)
     if not args.no_cpu:
         grids = {"fp32c": precision_grids(hpg, hpg.GridMode.FP32_COMPENSATED, dev_records, cf),
                  "fp64": precision_grids(hpg, hpg.GridMode.FP64, dev_records, cf)}

@@ -106,6 +106,17 @@ struct hpgx_state_s {
 
   CallBuffers cb{};
   size_t vis_capacity = 0;  // records
+  // tasks (max_added_tasks + 1): record buffers that host batches are
+  // uploaded into on the task's own copy stream, in rotation, so that the
+  // upload of one call overlaps the kernels of the previous ones (all
+  // kernels stay on `stream`: they update the grid in place).  Task 0's
+  // buffer is cb.vis as allocated; cb.vis points to the current task's.
+  unsigned tasks = 1;
+  unsigned task = 0;
+  std::vector<void*> task_vis;
+  std::vector<hipStream_t> copy_streams;
+  std::vector<hipEvent_t> copied, released;  // per task
+  std::vector<bool> released_set;
   // gridding pipeline: 5 = as 4, but FP32 grids with (hi, lo) pair tiles
   // are added by one wave per tile quadrant with one lane per CF tap and no
   // LDS atomics (default; 4 where the quadrant regions do not fit LDS), 4 = block-local
@@ -419,6 +430,17 @@ void free_state(hpgx_state_t st) {
   dev_free(st->cf.wsum);
   dev_free(st->staging);
   CallBuffers& cb = st->cb;
+  for (size_t k = 1; k < st->task_vis.size(); ++k) dev_free(st->task_vis[k]);
+  if (!st->task_vis.empty()) cb.vis = st->task_vis[0];
+  for (hipStream_t cs : st->copy_streams)
+    if (cs) {
+      (void)hipStreamSynchronize(cs);
+      (void)hipStreamDestroy(cs);
+    }
+  for (hipEvent_t e : st->copied)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : st->released)
+    if (e) (void)hipEventDestroy(e);
   dev_free(cb.vis);
   dev_free(cb.gvis);
   dev_free(cb.hist);
@@ -809,6 +831,64 @@ int noalloc_plan(hpgx_state_t st, void* base, size_t n, CallBuffers& c, size_t& 
   return HPGX_SUCCESS;
 }
 
+// tasks a state keeps in rotation at most (max_added_tasks + 1 is clamped
+// to it; the reference clamps to its device's active_task_limit,
+// src/hpg_runtime.hpp:209-265, 758-770)
+constexpr unsigned task_limit = 16;
+
+int create_tasks(hpgx_state_t st, unsigned tasks) {
+  st->tasks = tasks;
+  st->task_vis.assign(tasks, nullptr);
+  st->task_vis[0] = st->cb.vis;
+  st->copy_streams.assign(tasks, nullptr);
+  st->copied.assign(tasks, nullptr);
+  st->released.assign(tasks, nullptr);
+  st->released_set.assign(tasks, false);
+  if (tasks < 2) return HPGX_SUCCESS;
+  for (unsigned k = 0; k < tasks; ++k) {
+    HPGX_HIP(st, hipStreamCreateWithFlags(&st->copy_streams[k], hipStreamNonBlocking));
+    HPGX_HIP(st, hipEventCreateWithFlags(&st->copied[k], hipEventDisableTiming));
+    HPGX_HIP(st, hipEventCreateWithFlags(&st->released[k], hipEventDisableTiming));
+  }
+  return HPGX_SUCCESS;
+}
+
+// the kernels queued so far may read the current task's record buffer: its
+// next upload waits for them
+int mark_task_use(hpgx_state_t st) {
+  if (st->tasks < 2) return HPGX_SUCCESS;
+  HPGX_HIP(st, hipEventRecord(st->released[st->task], st->stream));
+  st->released_set[st->task] = true;
+  return HPGX_SUCCESS;
+}
+
+// host records of the next call: uploaded into the next task's buffer on
+// that task's copy stream once the kernels that last read the buffer are
+// done; the state's stream waits for the upload.  The call returns when the
+// upload is complete (the caller may then reuse its host memory, pinned or
+// not), while the previous calls' kernels go on running.
+int upload_task_records(hpgx_state_t st, const void* vis, size_t n) {
+  const unsigned k = (st->task + 1) % st->tasks;
+  const size_t rec_bytes = visdata_size(st->npol);
+  if (!st->task_vis[k]) {
+    if (k == 0) {
+      st->task_vis[0] = st->cb.vis;
+    } else {
+      HPGX_HIP(st, dev_alloc(st->task_vis[k], std::max<size_t>(st->max_batch, 1) * rec_bytes));
+    }
+  }
+  if (st->released_set[k]) HPGX_HIP(st, hipStreamWaitEvent(st->copy_streams[k], st->released[k], 0));
+  if (n > 0)
+    HPGX_HIP(st, hipMemcpyAsync(st->task_vis[k], vis, n * rec_bytes, hipMemcpyHostToDevice,
+                                st->copy_streams[k]));
+  HPGX_HIP(st, hipEventRecord(st->copied[k], st->copy_streams[k]));
+  HPGX_HIP(st, hipStreamWaitEvent(st->stream, st->copied[k], 0));
+  HPGX_HIP(st, hipEventSynchronize(st->copied[k]));
+  st->task = k;
+  st->cb.vis = st->task_vis[k];
+  return HPGX_SUCCESS;
+}
+
 int stage_records(hpgx_state_t st, const void* vis, size_t n, int memory) {
   const size_t bytes = n * visdata_size(st->npol);
   if (bytes == 0) return HPGX_SUCCESS;
@@ -902,9 +982,8 @@ int hpgx_create(
   int dev = g_device;
   if (dev < 0) (void)hipGetDevice(&dev);
   st->device = dev;
-  // max_added_tasks is limited to 0: one stream per state
-  (void)max_added_tasks;
-  st->max_added_tasks = 0;
+  // max_added_tasks + 1 tasks, at most task_limit (reported back clamped)
+  st->max_added_tasks = std::min(max_added_tasks, task_limit - 1);
   st->max_batch = max_visibility_batch_size;
   std::memcpy(st->gsize, grid_size, sizeof(st->gsize));
   st->gscale[0] = grid_scale[0];
@@ -944,6 +1023,7 @@ int hpgx_create(
     rc = hpgx_allocate_convolution_function_region(
       st, init_cf_oversampling, init_cf_num_groups, init_cf_extents);
   if (!rc) rc = ensure_call_buffers(st);
+  if (!rc) rc = create_tasks(st, st->max_added_tasks + 1);
   if (!rc) {
     e = hipStreamSynchronize(st->stream);
     if (e != hipSuccess) rc = hip_fail(st, e, "state initialization");
@@ -1240,6 +1320,10 @@ int hpgx_grid_visibilities(
   } else if (dev && binned_degrid) {
     records = vis;
     if (return_visibilities) out_records = vis_out;
+  } else if (!dev && st->tasks > 1) {
+    // uploaded on the next task's copy stream: overlaps the kernels of the
+    // calls before
+    if (int rc = upload_task_records(st, vis, n)) return rc;
   } else {
     if (int rc = stage_records(st, vis, n, memory)) return rc;
   }
@@ -1253,7 +1337,7 @@ int hpgx_grid_visibilities(
                    memory == HPGX_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                    st->stream));
   }
-  return HPGX_SUCCESS;
+  return mark_task_use(st);
 }
 
 int hpgx_grid_visibilities_noalloc(
@@ -1311,6 +1395,7 @@ int hpgx_degrid_adjoint(
     (void)hipStreamSynchronize(st->stream);
     dev_free(tmp);
   }
+  if (!rc) rc = mark_task_use(st);
   return rc;
 }
 
@@ -1356,13 +1441,15 @@ int hpgx_degrid_forward(
     HPGX_HIP(st, hipStreamSynchronize(st->stream));
     dev_free(tmp);
   }
-  return HPGX_SUCCESS;
+  return mark_task_use(st);
 }
 
 int hpgx_fence(hpgx_state_t st) {
   if (!st) return HPGX_ERR_OTHER;
   if (int rc = set_device(st)) return rc;
   HPGX_HIP(st, hipStreamSynchronize(st->stream));
+  for (hipStream_t cs : st->copy_streams)
+    if (cs) HPGX_HIP(st, hipStreamSynchronize(cs));
   unsigned long long invalid = 0;
   if (st->cb.invalid_count) {
     HPGX_HIP(st, hipMemcpy(&invalid, st->cb.invalid_count, sizeof(invalid), hipMemcpyDeviceToHost));

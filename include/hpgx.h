@@ -100,9 +100,14 @@ HPGX_API const char* hpgx_version(void);
 
 /* hpg::GridderState::create (src/hpg.cc:226-264) +
  * StateT ctor (src/hpg_runtime.hpp:757-781): allocates the zeroed grid and
- * weights.  mueller_indexes / conjugate_mueller_indexes: num_mueller_rows x
- * num_polarizations ints, row-major; num_mueller_rows must equal
- * grid_size[2].  init_cf_extents (may be null): init_cf_num_groups x 4
+ * weights.  max_added_tasks + 1 tasks (clamped to 16; hpgx_max_added_tasks
+ * reports the clamped value) rotate over host batches: each uploads its
+ * records into its own device buffer on its own stream, so that the upload
+ * of one grid_visibilities call overlaps the kernels of the calls before
+ * (the rotation of src/hpg_runtime.hpp:1643-1681); all kernels run in call
+ * order on the state's stream.  mueller_indexes /
+ * conjugate_mueller_indexes: num_mueller_rows x num_polarizations ints,
+ * row-major; num_mueller_rows must equal grid_size[2].  init_cf_extents (may be null): init_cf_num_groups x 4
  * extents used to size the first CF region. */
 HPGX_API int hpgx_create(
   hpgx_state_t* state, unsigned max_added_tasks,
