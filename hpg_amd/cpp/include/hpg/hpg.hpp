@@ -342,6 +342,42 @@ public:
             ((ext[1] - 2 * padding * os) / os) / 2};
   }
   virtual ~CFArray() {}
+
+  // copy the values of group grp into dst, in the layout `device` takes
+  // them without a host-side re-layout, returning that layout's version
+  // string (src/hpg.hpp:697-720, src/hpg.cc:1566-1612).  For Device::Hip
+  // the layout is the CFArray order, x fastest, then y, mueller, cube
+  // (Layout::Left): the device lays its CF out for gridding itself, on the
+  // GPU; a DeviceCFArray saves the per-tap calls of operator().
+  rval_t<std::string> copy_to(Device device, Device host_device, unsigned grp,
+                              value_type* dst) const;
+  // elements of the destination of copy_to()
+  rval_t<size_t> min_buffer_size(Device device, unsigned grp) const;
+};
+
+// version string of CF layouts with no device (src/hpg.cc:1562-1563)
+extern const char* const cf_layout_unspecified_version;
+
+// CF values stored in a device's layout (src/hpg.hpp:723-751): created from
+// copy_to() buffers; set_convolution_function() hands them over unchanged
+class DeviceCFArray : public CFArray {
+public:
+  static rval_t<std::unique_ptr<DeviceCFArray>> create(
+    const std::string& version, unsigned oversampling,
+    std::vector<std::tuple<std::array<unsigned, rank - 1>, std::vector<value_type>>>&& arrays);
+  virtual Device device() const = 0;
+  virtual ~DeviceCFArray() {}
+};
+
+// writable DeviceCFArray (src/hpg.hpp:753-790), zero-initialized to a shape
+class RWDeviceCFArray : public DeviceCFArray {
+public:
+  static rval_t<std::unique_ptr<RWDeviceCFArray>> create(Device device,
+                                                         const CFArrayShape& shape);
+  using CFArray::operator();
+  virtual std::complex<cf_fp>& operator()(unsigned x, unsigned y, unsigned mueller,
+                                          unsigned cube, unsigned group) = 0;
+  virtual ~RWDeviceCFArray() {}
 };
 
 // ---- grid values and weights (src/hpg.hpp:796-960) ---------------------------
@@ -382,6 +418,11 @@ public:
 
   opt_t<Error> copy_to(Device host_device, value_type* dst, Layout layout = Layout::Left) const;
   virtual size_t min_buffer_size() const { return size_t(extent(0)) * extent(1); }
+  // src/hpg.hpp:940-950: weights [mrow][cube] from a buffer in `layout`
+  static std::unique_ptr<GridWeightArray> copy_from(
+    const std::string& name, Device target_device, Device host_device,
+    const value_type* src, const std::array<unsigned, rank>& extents,
+    Layout layout = Layout::Left);
   virtual ~GridWeightArray() {}
 };
 

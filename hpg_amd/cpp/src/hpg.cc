@@ -106,7 +106,119 @@ public:
   }
 };
 
+// ---- CF layouts (src/hpg_impl.cc:77-141) -----------------------------------
+
+// version number of the Hip device's CF layout: the CFArray order, x
+// fastest, then y, mueller, cube (Layout::Left over the CFArray axes), which
+// hpgx_set_convolution_function takes and lays out for gridding on the GPU
+constexpr unsigned cf_layout_version_number = 1;
+const char* const hip_device_name = "Hip";
+
+std::string hip_cf_layout_version() {
+  return std::to_string(cf_layout_version_number) + "-" + hip_device_name;
+}
+
+// "<version number>-<device name>" -> (number, device name)
+std::optional<std::pair<unsigned, std::string>> parse_cf_layout(const std::string& layout) {
+  const auto dash = layout.find('-');
+  if (dash == std::string::npos || dash == 0) return std::nullopt;
+  unsigned vn = 0;
+  for (size_t i = 0; i < dash; ++i) {
+    if (layout[i] < '0' || layout[i] > '9') return std::nullopt;
+    vn = vn * 10 + unsigned(layout[i] - '0');
+  }
+  return std::make_pair(vn, layout.substr(dash + 1));
+}
+
+// CF values in the Hip layout, one array per group
+class HipDeviceCFArray final : public RWDeviceCFArray {
+public:
+  unsigned m_os;
+  std::vector<std::array<unsigned, 4>> m_ext;
+  std::vector<std::vector<value_type>> m_v;
+
+  HipDeviceCFArray(unsigned os,
+                   std::vector<std::tuple<std::array<unsigned, 4>, std::vector<value_type>>>&& a)
+    : m_os(os) {
+    for (auto& [e, v] : a) {
+      m_ext.push_back(e);
+      m_v.push_back(std::move(v));
+    }
+  }
+  explicit HipDeviceCFArray(const CFArrayShape& shape) : m_os(shape.oversampling()) {
+    for (unsigned g = 0; g < shape.num_groups(); ++g) {
+      const auto e = shape.extents(g);
+      m_ext.push_back(e);
+      m_v.emplace_back(size_t(e[0]) * e[1] * e[2] * e[3], value_type(0));
+    }
+  }
+  unsigned oversampling() const override { return m_os; }
+  unsigned num_groups() const override { return unsigned(m_ext.size()); }
+  std::array<unsigned, 4> extents(unsigned g) const override { return m_ext[g]; }
+  Device device() const override { return Device::Hip; }
+  std::complex<cf_fp> operator()(unsigned x, unsigned y, unsigned m, unsigned c,
+                                 unsigned g) const override {
+    return m_v[g][left_index(m_ext[g], x, y, m, c)];
+  }
+  std::complex<cf_fp>& operator()(unsigned x, unsigned y, unsigned m, unsigned c,
+                                  unsigned g) override {
+    return m_v[g][left_index(m_ext[g], x, y, m, c)];
+  }
+  const value_type* data(unsigned g) const { return m_v[g].data(); }
+};
+
 }  // namespace
+
+const char* const cf_layout_unspecified_version = "";
+
+rval_t<std::string> CFArray::copy_to(Device device, Device host_device, unsigned grp,
+                                     value_type* dst) const {
+  // src/hpg.cc:1566-1606
+  if (host_devices().count(host_device) == 0) return rval<std::string>(DisabledHostDeviceError());
+  if (devices().count(device) == 0) return rval<std::string>(DisabledDeviceError());
+  const auto e = extents(grp);
+  const long planes = long(e[2]) * e[3];
+#pragma omp parallel for if (host_device == Device::OpenMP) schedule(static)
+  for (long pc = 0; pc < planes; ++pc) {
+    const unsigned m = unsigned(pc % e[2]);
+    const unsigned c = unsigned(pc / e[2]);
+    for (unsigned y = 0; y < e[1]; ++y)
+      for (unsigned x = 0; x < e[0]; ++x) dst[left_index(e, x, y, m, c)] = (*this)(x, y, m, c, grp);
+  }
+  return rval(hip_cf_layout_version());
+}
+
+rval_t<size_t> CFArray::min_buffer_size(Device device, unsigned grp) const {
+  // src/hpg.cc:1608-1612
+  if (devices().count(device) == 0) return rval<size_t>(DisabledDeviceError());
+  const auto e = extents(grp);
+  return rval(size_t(e[0]) * e[1] * e[2] * e[3]);
+}
+
+rval_t<std::unique_ptr<DeviceCFArray>> DeviceCFArray::create(
+  const std::string& version, unsigned oversampling,
+  std::vector<std::tuple<std::array<unsigned, rank - 1>, std::vector<value_type>>>&& arrays) {
+  // src/hpg.cc:1614-1660: the layout must name an enabled device exactly
+  using R = rval_t<std::unique_ptr<DeviceCFArray>>;
+  const auto parsed = parse_cf_layout(version);
+  if (!parsed) return R(Error("Provided layout is invalid", ErrorType::InvalidCFLayout));
+  if (parsed->second != hip_device_name || devices().count(Device::Hip) == 0)
+    return R(DisabledDeviceError());
+  if (parsed->first != cf_layout_version_number)
+    return R(Error("Provided layout version is not supported", ErrorType::InvalidCFLayout));
+  for (auto& [e, v] : arrays)
+    if (v.size() != size_t(e[0]) * e[1] * e[2] * e[3])
+      return R(Error("CF values do not match their extents", ErrorType::InvalidCFLayout));
+  return R(std::unique_ptr<DeviceCFArray>(new HipDeviceCFArray(oversampling, std::move(arrays))));
+}
+
+rval_t<std::unique_ptr<RWDeviceCFArray>> RWDeviceCFArray::create(Device device,
+                                                                 const CFArrayShape& shape) {
+  // src/hpg.cc:1662-1701
+  using R = rval_t<std::unique_ptr<RWDeviceCFArray>>;
+  if (device != Device::Hip || devices().count(device) == 0) return R(DisabledDeviceError());
+  return R(std::unique_ptr<RWDeviceCFArray>(new HipDeviceCFArray(shape)));
+}
 
 // ---- runtime::State: one device state ---------------------------------
 
@@ -238,6 +350,17 @@ opt_t<Error> GridWeightArray::copy_to(Device host_device, value_type* dst, Layou
   return std::nullopt;
 }
 
+std::unique_ptr<GridWeightArray> GridWeightArray::copy_from(
+  const std::string&, Device, Device, const value_type* src,
+  const std::array<unsigned, rank>& extents, Layout layout) {
+  auto a = std::make_unique<HostGridWeightArray>(extents);
+  for (unsigned c = 0; c < extents[1]; ++c)
+    for (unsigned r = 0; r < extents[0]; ++r)
+      (*a)(r, c) =
+        src[layout == Layout::Left ? size_t(c) * extents[0] + r : size_t(r) * extents[1] + c];
+  return a;
+}
+
 // ---- GridderState ----------------------------------------------------------
 
 GridderState::GridderState() {}
@@ -326,10 +449,21 @@ opt_error_t do_set_cf(runtime::State& st, Device host_device, const CFArray& cf)
     if (size_t(ext[4 * g]) > size_t(st.grid_size[0]) * cf.oversampling()
         || size_t(ext[4 * g + 1]) > size_t(st.grid_size[1]) * cf.oversampling())
       return Error("CF support size exceeds grid size", ErrorType::CFSupportExceedsGrid);
-  // CFArray -> Layout::Left (x, y, mueller, cube), one call per tap, as
-  // init_cf_host does (src/hpg_impl.hpp:1035-1046)
   std::vector<std::vector<std::complex<float>>> vals(ng);
   std::vector<const void*> ptrs(ng);
+  // a DeviceCFArray in this device's layout is handed over as it is
+  // (CFPool::add_device_cfs, src/hpg_runtime.hpp:513-551, taken through the
+  // dynamic_cast of src/hpg_runtime.hpp:907-913)
+  if (auto* d = dynamic_cast<const HipDeviceCFArray*>(&cf)) {
+    for (unsigned g = 0; g < ng; ++g) ptrs[g] = d->data(g);
+    int rc = hpgx_set_convolution_function(
+      st.h, cf.oversampling(), ng, ext.empty() ? nullptr : ext.data(),
+      ptrs.empty() ? nullptr : ptrs.data(), HPGX_HOST);
+    if (rc) return hpgx_error(st.h, rc);
+    return std::nullopt;
+  }
+  // CFArray -> Layout::Left (x, y, mueller, cube), one call per tap, as
+  // init_cf_host does (src/hpg_impl.hpp:1035-1046)
   for (unsigned g = 0; g < ng; ++g) {
     const unsigned* e = &ext[4 * g];
     auto& v = vals[g];

@@ -5,6 +5,7 @@
 #include "hpg/hpg.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -386,6 +387,329 @@ void test_shift_cycle_and_fft() {
   CHECK((*modelf)(1070, 10, 0, 0).imag() < 0);
 }
 
+// CF whose tap (x, y, mueller) holds the value x + 1000 y + i mueller
+// (EncodedCFArray, DeviceCFArrayUnitTests.cc:34-90)
+struct EncodedCF final : public hpg::CFArray {
+  unsigned m_nm, m_os, m_radius;
+  EncodedCF(unsigned nm, unsigned os, unsigned radius) : m_nm(nm), m_os(os), m_radius(radius) {}
+  unsigned oversampling() const override { return m_os; }
+  unsigned num_groups() const override { return 1; }
+  std::array<unsigned, 4> extents(unsigned) const override {
+    const unsigned w = (2 * m_radius + 1 + 2 * padding) * m_os;
+    return {w, w, m_nm, 1};
+  }
+  std::complex<float> operator()(unsigned x, unsigned y, unsigned m, unsigned,
+                                 unsigned) const override {
+    return {float(x + 1000 * y), float(m)};
+  }
+};
+
+// records uniform over the part of a grid where supports of radius r fit
+std::vector<hpg::VisData<1>> random_vis(unsigned n, const std::array<unsigned, 4>& gsz,
+                                        const std::array<double, 2>& scale, unsigned r,
+                                        unsigned ncube, unsigned seed) {
+  std::mt19937 rng(seed);
+  std::uniform_real_distribution<double> ux(-(gsz[0] / 2.0 - r - 2), gsz[0] / 2.0 - r - 2);
+  std::uniform_real_distribution<double> uy(-(gsz[1] / 2.0 - r - 2), gsz[1] / 2.0 - r - 2);
+  std::uniform_real_distribution<float> dv(-1, 1);
+  std::vector<hpg::VisData<1>> v;
+  for (unsigned i = 0; i < n; ++i)
+    v.emplace_back(std::array<std::complex<float>, 1>{std::complex<float>(dv(rng), dv(rng))},
+                   std::array<float, 1>{0.5f + 0.5f * dv(rng)}, 299792458.0, 0.0,
+                   hpg::vis_uvw_t{ux(rng) / scale[0], uy(rng) / scale[1], 0.0},
+                   unsigned(i % ncube), std::array<unsigned, 2>{0, 0});
+  return v;
+}
+
+// max |a - b| / max |a| over every element
+double rel_diff(const hpg::GridValueArray& a, const hpg::GridValueArray& b) {
+  double mx = 0, d = 0;
+  for (unsigned c = 0; c < a.extent(3); ++c)
+    for (unsigned r = 0; r < a.extent(2); ++r)
+      for (unsigned y = 0; y < a.extent(1); ++y)
+        for (unsigned x = 0; x < a.extent(0); ++x) {
+          mx = std::max(mx, std::abs(a(x, y, r, c)));
+          d = std::max(d, std::abs(a(x, y, r, c) - b(x, y, r, c)));
+        }
+  return mx > 0 ? d / mx : (d == 0 ? 0.0 : 1.0);
+}
+
+void test_device_cf_layout() {
+  // DeviceCFArrayUnitTests.cc:384-428 for this device's layout: the buffer
+  // of copy_to() holds tap (x, y, mueller) at x + ext_x (y + ext_y mueller)
+  EncodedCF cf(4, 3, 2);
+  auto sz = hpg::get_value(cf.min_buffer_size(dev, 0));
+  const auto e = cf.extents(0);
+  CHECK(sz == size_t(e[0]) * e[1] * e[2] * e[3]);
+  std::vector<hpg::CFArray::value_type> a(sz);
+  auto vsn = cf.copy_to(dev, host, 0, a.data());
+  CHECK(hpg::is_value(vsn));
+  bool ok = true;
+  for (size_t i = 0; i < a.size(); ++i) {
+    const unsigned x = unsigned(i % e[0]);
+    const unsigned y = unsigned((i / e[0]) % e[1]);
+    const unsigned m = unsigned(i / (size_t(e[0]) * e[1]));
+    ok &= a[i] == std::complex<float>(float(x + 1000 * y), float(m));
+  }
+  CHECK(ok);
+  // OpenMP host device: the same buffer
+  std::vector<hpg::CFArray::value_type> b(sz);
+  CHECK(hpg::is_value(cf.copy_to(dev, hpg::Device::OpenMP, 0, b.data())));
+  CHECK(a == b);
+}
+
+void test_device_cf_layout_version() {
+  EncodedCF cf(2, 4, 1);
+  std::vector<hpg::CFArray::value_type> a(hpg::get_value(cf.min_buffer_size(dev, 0)));
+  auto vsn = hpg::get_value(cf.copy_to(dev, host, 0, a.data()));
+  CHECK(vsn == "1-Hip");
+  auto arrays = [&]() {
+    std::vector<std::tuple<std::array<unsigned, 4>, std::vector<hpg::CFArray::value_type>>> v;
+    v.emplace_back(cf.extents(0), a);
+    return v;
+  };
+  auto ok = hpg::DeviceCFArray::create(vsn, 4, arrays());
+  CHECK(hpg::is_value(ok));
+  if (hpg::is_value(ok)) {
+    auto d = hpg::get_value(std::move(ok));
+    CHECK(d->device() == dev);
+    CHECK((*d)(5, 7, 1, 0, 0) == cf(5, 7, 1, 0, 0));
+  }
+  auto bad = hpg::DeviceCFArray::create("Hip", 4, arrays());
+  CHECK(hpg::is_error(bad) && hpg::get_error(bad).type() == hpg::ErrorType::InvalidCFLayout);
+  auto other = hpg::DeviceCFArray::create("1-Cuda", 4, arrays());
+  CHECK(hpg::is_error(other) && hpg::get_error(other).type() == hpg::ErrorType::DisabledDevice);
+  auto newer = hpg::DeviceCFArray::create("2-Hip", 4, arrays());
+  CHECK(hpg::is_error(newer) && hpg::get_error(newer).type() == hpg::ErrorType::InvalidCFLayout);
+  auto host_dev = cf.min_buffer_size(hpg::Device::Serial, 0);
+  CHECK(hpg::is_error(host_dev) && hpg::get_error(host_dev).type() == hpg::ErrorType::DisabledDevice);
+  auto no_host = cf.copy_to(dev, dev, 0, a.data());
+  CHECK(hpg::is_error(no_host)
+        && hpg::get_error(no_host).type() == hpg::ErrorType::DisabledHostDevice);
+  auto rw_host = hpg::RWDeviceCFArray::create(hpg::Device::Serial, cf);
+  CHECK(hpg::is_error(rw_host) && hpg::get_error(rw_host).type() == hpg::ErrorType::DisabledDevice);
+}
+
+// the grid of 1000 records on a 50^2 grid (scale 0.1, -0.1) with a CF given
+// as a CFArray and as a DeviceCFArray / RWDeviceCFArray of the same values
+// (DeviceCFArrayUnitTests.cc:458-547, RWGridding): FP64 grids, equal to
+// 1e-12 of max|grid| (the tile kernels add with LDS atomics, whose order
+// varies from run to run)
+void test_device_cf_gridding() {
+  const std::array<unsigned, 4> gsz{50, 50, 1, 1};
+  const std::array<double, 2> sc{0.1, -0.1};
+  ConeCF cf(1, 20, 10);
+  auto vis = random_vis(1000, gsz, sc, 10, 1, 42);
+  std::vector<hpg::CFArray::value_type> a(hpg::get_value(cf.min_buffer_size(dev, 0)));
+  auto vsn = hpg::get_value(cf.copy_to(dev, host, 0, a.data()));
+  std::vector<std::tuple<std::array<unsigned, 4>, std::vector<hpg::CFArray::value_type>>> arr;
+  arr.emplace_back(cf.extents(0), std::move(a));
+  auto devcf = hpg::get_value(hpg::DeviceCFArray::create(vsn, 20, std::move(arr)));
+  auto rw = hpg::get_value(hpg::RWDeviceCFArray::create(dev, cf));
+  const auto e = cf.extents(0);
+  for (unsigned y = 0; y < e[1]; ++y)
+    for (unsigned x = 0; x < e[0]; ++x) (*rw)(x, y, 0, 0, 0) = cf(x, y, 0, 0, 0);
+  auto grid_with = [&](hpg::CFArray&& c) {
+    auto gs = hpg::get_value(hpg::GridderState::create<1>(dev, 0, 1000, &c, gsz, sc, {{0}}, {{0}}));
+    gs = hpg::get_value(std::move(gs).set_convolution_function(host, std::move(c)));
+    gs = hpg::get_value(
+      std::move(gs).grid_visibilities(host, std::vector<hpg::VisData<1>>(vis)));
+    return std::get<1>(std::move(gs).grid_values());
+  };
+  auto g_cf = grid_with(ConeCF(cf));
+  CHECK(any_nonzero(*g_cf));
+  auto g_dev = grid_with(std::move(*devcf));
+  CHECK(rel_diff(*g_cf, *g_dev) <= 1e-12);
+  auto g_rw = grid_with(std::move(*rw));
+  CHECK(rel_diff(*g_cf, *g_rw) <= 1e-12);
+}
+
+// a CF with costly per-tap access (LargeCFArray, DeviceCFArrayUnitTests.cc:
+// 92-145): two groups of 4 Mueller elements on 1330^2 oversampled taps
+struct LargeCF final : public hpg::CFArray {
+  unsigned m_groups;
+  explicit LargeCF(unsigned groups) : m_groups(groups) {}
+  unsigned oversampling() const override { return 10; }
+  unsigned num_groups() const override { return m_groups; }
+  std::array<unsigned, 4> extents(unsigned) const override { return {1330, 1330, 4, 1}; }
+  std::complex<float> operator()(unsigned x, unsigned y, unsigned m, unsigned,
+                                 unsigned g) const override {
+    const float fx = float(x) - 665.0f, fy = float(y) - 665.0f;
+    return std::polar(1.0f / (1.0f + 1e-4f * (fx * fx + fy * fy)), 1e-3f * fx * fy * (m + g + 1));
+  }
+};
+
+void test_device_cf_efficiency() {
+  // DeviceCFArrayUnitTests.cc:549-659: setting ten copies of a CF is faster
+  // from DeviceCFArrays (values handed over) than from CFArrays (one call
+  // per tap)
+  LargeCF cf(2);
+  std::vector<std::tuple<std::array<unsigned, 4>, std::vector<hpg::CFArray::value_type>>> arr;
+  std::string vsn;
+  for (unsigned g = 0; g < cf.num_groups(); ++g) {
+    std::vector<hpg::CFArray::value_type> a(hpg::get_value(cf.min_buffer_size(dev, g)));
+    vsn = hpg::get_value(cf.copy_to(dev, host, g, a.data()));
+    arr.emplace_back(cf.extents(g), std::move(a));
+  }
+  auto time_set_cf = [&](std::vector<std::unique_ptr<hpg::CFArray>>& cfs) {
+    auto gs = hpg::get_value(hpg::GridderState::create<1>(dev, 0, 10, cfs[0].get(), {1000, 1000, 1, 1},
+                                                          {0.1, 0.1}, {{0}}, {{0}}));
+    gs = std::move(gs).fence();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (auto& c : cfs)
+      gs = hpg::get_value(std::move(gs).set_convolution_function(host, std::move(*c)));
+    gs = std::move(gs).fence();
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
+  double t_cf, t_devcf;
+  {
+    std::vector<std::unique_ptr<hpg::CFArray>> cfs;
+    for (int i = 0; i < 10; ++i) cfs.emplace_back(new LargeCF(cf));
+    t_cf = time_set_cf(cfs);
+  }
+  {
+    std::vector<std::unique_ptr<hpg::CFArray>> cfs;
+    for (int i = 0; i < 10; ++i)
+      cfs.push_back(hpg::get_value(hpg::DeviceCFArray::create(vsn, 10, decltype(arr)(arr))));
+    t_devcf = time_set_cf(cfs);
+  }
+  std::printf("  cf %.3f s; devcf %.3f s\n", t_cf, t_devcf);
+  CHECK(t_devcf < t_cf);
+}
+
+// ---- grid value and weight arrays (GridArraysUnitTests.cc:266-709) ---------
+
+hpg::GridValueArray::value_type grid_value_encode(unsigned x, unsigned y, unsigned mr,
+                                                  unsigned cb) {
+  return {double(x * 1000 + y), double(mr * 1000 + cb)};
+}
+double grid_weight_encode(unsigned mr, unsigned cb) { return double(1000 * mr) + double(cb); }
+
+void test_grid_arrays() {
+  const std::array<unsigned, 4> gsz{8, 9, 2, 3};
+  RandomCF cf(10, {{3 + 4, 3 + 4, 2, 3}, {2 + 4, 2 + 4, 2, 2}}, 42);
+  auto gs = hpg::get_value(
+    hpg::GridderState::create<1>(dev, 0, 4, &cf, gsz, {0.1, -0.1}, {{0}, {0}}, {{0}, {0}}));
+  // GridValueReadWrite, CopyToValuesLayouts
+  auto gvals = std::get<1>(gs.grid_values());
+  for (unsigned x = 0; x < gsz[0]; ++x)
+    for (unsigned y = 0; y < gsz[1]; ++y)
+      for (unsigned mr = 0; mr < gsz[2]; ++mr)
+        for (unsigned cb = 0; cb < gsz[3]; ++cb) (*gvals)(x, y, mr, cb) = grid_value_encode(x, y, mr, cb);
+  bool rw = true, left = true, right = true;
+  const size_t n = gvals->min_buffer_size();
+  CHECK(n == size_t(gsz[0]) * gsz[1] * gsz[2] * gsz[3]);
+  std::vector<hpg::GridValueArray::value_type> vl(n), vr(n);
+  CHECK(!gvals->copy_to(host, vl.data(), hpg::Layout::Left));
+  CHECK(!gvals->copy_to(host, vr.data(), hpg::Layout::Right));
+  for (unsigned x = 0; x < gsz[0]; ++x)
+    for (unsigned y = 0; y < gsz[1]; ++y)
+      for (unsigned mr = 0; mr < gsz[2]; ++mr)
+        for (unsigned cb = 0; cb < gsz[3]; ++cb) {
+          const auto val = grid_value_encode(x, y, mr, cb);
+          rw &= (*gvals)(x, y, mr, cb) == val;
+          left &= vl[x + gsz[0] * (y + gsz[1] * (mr + gsz[2] * cb))] == val;
+          right &= vr[cb + gsz[3] * (mr + gsz[2] * (y + gsz[1] * x))] == val;
+        }
+  CHECK(rw && left && right);
+  // CopyFromValuesLayouts: from each layout, then back through the state
+  const std::array<unsigned, 4> g2{8, 9, 4, 3};
+  for (auto layout : {hpg::Layout::Left, hpg::Layout::Right}) {
+    std::vector<hpg::GridValueArray::value_type> src(size_t(g2[0]) * g2[1] * g2[2] * g2[3]);
+    for (unsigned x = 0; x < g2[0]; ++x)
+      for (unsigned y = 0; y < g2[1]; ++y)
+        for (unsigned mr = 0; mr < g2[2]; ++mr)
+          for (unsigned cb = 0; cb < g2[3]; ++cb)
+            src[layout == hpg::Layout::Left ? x + g2[0] * (y + g2[1] * (mr + g2[2] * cb))
+                                            : cb + g2[3] * (mr + g2[2] * (y + g2[1] * x))] =
+              grid_value_encode(x, y, mr, cb);
+    auto a = hpg::GridValueArray::copy_from("v", dev, host, src.data(), g2, layout);
+    bool eq = true;
+    for (unsigned x = 0; x < g2[0]; ++x)
+      for (unsigned y = 0; y < g2[1]; ++y)
+        for (unsigned mr = 0; mr < g2[2]; ++mr)
+          for (unsigned cb = 0; cb < g2[3]; ++cb) eq &= (*a)(x, y, mr, cb) == grid_value_encode(x, y, mr, cb);
+    CHECK(eq);
+    // the model of a state set from it reads back the same values
+    RandomCF cf4(10, {{3 + 4, 3 + 4, 4, 3}}, 43);
+    auto s4 = hpg::get_value(hpg::GridderState::create<1>(
+      dev, 0, 4, &cf4, g2, {0.1, -0.1}, {{0}, {0}, {0}, {0}}, {{0}, {0}, {0}, {0}}));
+    s4 = hpg::get_value(std::move(s4).set_model(host, std::move(*a)));
+    auto m = std::get<1>(std::move(s4).model_values());
+    bool meq = true;
+    for (unsigned x = 0; x < g2[0]; ++x)
+      for (unsigned y = 0; y < g2[1]; ++y)
+        for (unsigned mr = 0; mr < g2[2]; ++mr)
+          for (unsigned cb = 0; cb < g2[3]; ++cb) meq &= (*m)(x, y, mr, cb) == grid_value_encode(x, y, mr, cb);
+    CHECK(meq);
+  }
+  // CopyFromWeightsLayouts
+  const std::array<unsigned, 2> w2{4, 3};
+  for (auto layout : {hpg::Layout::Left, hpg::Layout::Right}) {
+    std::vector<double> src(size_t(w2[0]) * w2[1]);
+    for (unsigned mr = 0; mr < w2[0]; ++mr)
+      for (unsigned cb = 0; cb < w2[1]; ++cb)
+        src[layout == hpg::Layout::Left ? mr + w2[0] * cb : cb + w2[1] * mr] = grid_weight_encode(mr, cb);
+    auto a = hpg::GridWeightArray::copy_from("w", dev, host, src.data(), w2, layout);
+    bool eq = a->extent(0) == w2[0] && a->extent(1) == w2[1];
+    for (unsigned mr = 0; mr < w2[0]; ++mr)
+      for (unsigned cb = 0; cb < w2[1]; ++cb) eq &= (*a)(mr, cb) == grid_weight_encode(mr, cb);
+    CHECK(eq);
+  }
+  // CopyToWeightsLayouts
+  auto gw = std::get<1>(gs.grid_weights());
+  for (unsigned mr = 0; mr < gsz[2]; ++mr)
+    for (unsigned cb = 0; cb < gsz[3]; ++cb) (*gw)(mr, cb) = grid_weight_encode(mr, cb);
+  std::vector<double> wl(gw->min_buffer_size()), wr(gw->min_buffer_size());
+  CHECK(!gw->copy_to(host, wl.data(), hpg::Layout::Left));
+  CHECK(!gw->copy_to(host, wr.data(), hpg::Layout::Right));
+  bool weq = true;
+  for (unsigned mr = 0; mr < gsz[2]; ++mr)
+    for (unsigned cb = 0; cb < gsz[3]; ++cb) {
+      weq &= wl[mr + gsz[2] * cb] == grid_weight_encode(mr, cb);
+      weq &= wr[cb + gsz[3] * mr] == grid_weight_encode(mr, cb);
+    }
+  CHECK(weq);
+  CHECK(gvals->copy_to(dev, vl.data(), hpg::Layout::Left).has_value());  // not a host device
+}
+
+void test_compare_layouts() {
+  // GridArraysUnitTests.cc:580-709: the grid and weights of a gridding run,
+  // read in both layouts, agree element by element with each other and
+  // with the arrays
+  const std::array<unsigned, 4> gsz{20, 21, 1, 3};
+  const std::array<double, 2> sc{0.1, -0.1};
+  RandomCF cf(10, {{3 + 4, 3 + 4, 1, 3}, {2 + 4, 2 + 4, 1, 2}}, 42);
+  auto vis = random_vis(100, gsz, sc, 3, 3, 7);
+  for (size_t i = 0; i < vis.size(); ++i) vis[i].m_cf_index = {unsigned(i % 2), unsigned(i % 2)};
+  auto gs = hpg::get_value(hpg::GridderState::create<1>(dev, 0, 100, &cf, gsz, sc, {{0}}, {{0}}));
+  gs = hpg::get_value(std::move(gs).set_convolution_function(host, RandomCF(cf)));
+  gs = hpg::get_value(std::move(gs).grid_visibilities(host, std::move(vis), true));
+  auto [gs1, gvals] = std::move(gs).grid_values();
+  auto [gs2, gwgts] = std::move(gs1).grid_weights();
+  CHECK(any_nonzero(*gvals));
+  std::vector<hpg::GridValueArray::value_type> vl(gvals->min_buffer_size()), vr(vl.size());
+  std::vector<double> wl(gwgts->min_buffer_size()), wr(wl.size());
+  CHECK(!gvals->copy_to(host, vl.data(), hpg::Layout::Left));
+  CHECK(!gvals->copy_to(host, vr.data(), hpg::Layout::Right));
+  CHECK(!gwgts->copy_to(host, wl.data(), hpg::Layout::Left));
+  CHECK(!gwgts->copy_to(host, wr.data(), hpg::Layout::Right));
+  bool eq = true, wnz = false;
+  for (unsigned mr = 0; mr < gsz[2]; ++mr)
+    for (unsigned cb = 0; cb < gsz[3]; ++cb) {
+      const double a = wl[mr + gsz[2] * cb], b = wr[cb + gsz[3] * mr];
+      eq &= a == b && a == (*gwgts)(mr, cb);
+      wnz |= a != 0;
+      for (unsigned x = 0; x < gsz[0]; ++x)
+        for (unsigned y = 0; y < gsz[1]; ++y) {
+          const auto& l = vl[x + gsz[0] * (y + gsz[1] * (mr + gsz[2] * cb))];
+          const auto& r = vr[cb + gsz[3] * (mr + gsz[2] * (y + gsz[1] * x))];
+          eq &= l == r && l == (*gvals)(x, y, mr, cb);
+        }
+    }
+  CHECK(eq && wnz);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -402,6 +726,12 @@ int main(int argc, char** argv) {
   run("Errors", test_errors);
   run("CopiesAndSerialization", test_copy_semantics_and_serialization);
   run("ShiftCycleAndOneFFT", test_shift_cycle_and_fft);
+  run("DeviceCFArrayLayout", test_device_cf_layout);
+  run("DeviceCFArrayLayoutVersion", test_device_cf_layout_version);
+  run("DeviceCFArrayGridding", test_device_cf_gridding);
+  run("DeviceCFArrayEfficiency", test_device_cf_efficiency);
+  run("GridArrays", test_grid_arrays);
+  run("GridArraysCompareLayouts", test_compare_layouts);
   std::printf("%d checks, %d failures\n", g_checks, g_failures);
   return g_failures == 0 ? 0 : 1;
 }
