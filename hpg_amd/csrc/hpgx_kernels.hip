@@ -738,26 +738,58 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
   unsigned gc[bin_recs_per_thread];
   unsigned ninvalid = 0;
 
+  // grid rows with an index, per sign of w (bit gpol): the Mueller rows read
+  // at constant indices (kernel-argument scalars), not per record
+  unsigned rows_m = 0u, rows_mc = 0u;
+#pragma unroll
+  for (int gp = 0; gp < 4; ++gp) {
+    if (gp >= geo.nr) continue;
+    rows_m |= row_has_index(mu.M[gp], N) ? 1u << gp : 0u;
+    rows_mc |= row_has_index(mu.Mc[gp], N) ? 1u << gp : 0u;
+  }
+  const bool one_group = geo.ngroups == 1;
+  const GroupInfo group0 = groups[0];
+
+  // N == 1: the thread's records are loaded whole, two at a time, so that
+  // two records' loads are in flight at once rather than one record (and its
+  // chain of dependent loads) at a time; four would spill the 128 registers
+  // a 1024-thread block allows, larger records load per record
+  constexpr bool hoist = N == 1;
+  constexpr int words = int(sizeof(VisRec<N>) / sizeof(unsigned long long));
+  static_assert(sizeof(VisRec<N>) % sizeof(unsigned long long) == 0, "8-byte record words");
+  static_assert(bin_recs_per_thread % 2 == 0, "records loaded in pairs");
+  VisRec<N> rc[hoist ? 2 : 1];
+
 #pragma unroll
   for (int j = 0; j < bin_recs_per_thread; ++j) {
+    if constexpr (hoist) {
+      if (j % 2 == 0) {
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+          const size_t i = b * bin_records + size_t(j + jj) * bin_threads + threadIdx.x;
+          if (i >= n) continue;
+          const unsigned long long* src = reinterpret_cast<const unsigned long long*>(recs + i);
+          unsigned long long wds[words];
+#pragma unroll
+          for (int k = 0; k < words; ++k) wds[k] = src[k];
+          __builtin_memcpy(&rc[jj], wds, sizeof(VisRec<N>));
+        }
+      }
+    }
     keep[j] = false;
     const size_t i = b * bin_records + size_t(j) * bin_threads + threadIdx.x;
     if (i >= n) continue;
-    // N <= 2: the whole record in registers first, so its fields load in one
-    // round instead of one round per dependent use (larger records would
-    // spill the 128 registers a 1024-thread block allows)
+    // N == 2: the whole record in registers first, so its fields load in one
+    // round instead of one round per dependent use
     VisRec<N> r_copy;
-    if constexpr (N <= 2) {
-      constexpr int words = int(sizeof(VisRec<N>) / sizeof(unsigned long long));
-      static_assert(sizeof(VisRec<N>) % sizeof(unsigned long long) == 0,
-                    "8-byte record words");
+    if constexpr (N == 2) {
       const unsigned long long* src = reinterpret_cast<const unsigned long long*>(recs + i);
       unsigned long long wds[words];
 #pragma unroll
       for (int k = 0; k < words; ++k) wds[k] = src[k];
       __builtin_memcpy(&r_copy, wds, sizeof(r_copy));
     }
-    const VisRec<N>& r = N <= 2 ? r_copy : recs[i];
+    const VisRec<N>& r = hoist ? rc[j % 2] : N == 2 ? r_copy : recs[i];
     // degridding that returns the batch: every record is written to the
     // returned batch here, whose values the degridding kernels then replace
     if (copy_recs) {
@@ -771,13 +803,13 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
     const unsigned cfcube = r.cf_index[0];
     const unsigned gcube = r.grid_cube;
     bool valid = grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc);
-    if (valid) valid = cfcube < unsigned(groups[grp].ncube);
+    const GroupInfo g = !valid || one_group ? group0 : groups[grp];
+    if (valid) valid = cfcube < unsigned(g.ncube);
     if (!valid) {
       // counted by the gridding pass only
       if (!degrid_pass) ++ninvalid;
       continue;
     }
-    const GroupInfo g = groups[grp];
     const Coord c = vis_coord(r, geo, g);
     if (!c.inside) continue;
     const bool pos_w = r.uvw[2] > 0;
@@ -813,12 +845,22 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
       }
     }
     if (update_weights) {
-      // twgt = sum_v |sum_XY cf'| * w_v (src/hpg_core.hpp:877-885)
-      for (int gpol = 0; gpol < geo.nr; ++gpol) {
-        const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
+      // twgt = sum_v |sum_XY cf'| * w_v (src/hpg_core.hpp:877-885); grid
+      // rows and polarizations unrolled, so the Mueller indices are scalars
+      // and the slice-sum loads of all rows issue together
+      if (int(gcube) != wcube) {
+        if (wcube >= 0) flush_weights(wcube, wacc);
+        wcube = int(gcube);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wacc[k] = 0.0;
+      }
+#pragma unroll
+      for (int gp = 0; gp < 4; ++gp) {
+        if (gp >= geo.nr) continue;
         double t = 0.0;
+#pragma unroll
         for (int v = 0; v < N; ++v) {
-          const int m = row[v];
+          const int m = pos_w ? mu.M[gp][v] : mu.Mc[gp][v];
           double mag = 0.0;
           if (m >= 0 && m < g.nmueller) {
             const double2 sw = wsum[g.wsum_offset
@@ -828,21 +870,14 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
           }
           t += mag * double(r.w[v]);
         }
-        if (int(gcube) != wcube) {
-          if (wcube >= 0) flush_weights(wcube, wacc);
-          wcube = int(gcube);
-#pragma unroll
-          for (int k = 0; k < 4; ++k) wacc[k] = 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (k == gpol) wacc[k] += t;
+        wacc[gp] += t;
       }
     }
     const int tx0 = tiles_begin(c.gx), tx1 = HALO ? tx0 + 1 : tiles_end(c.gx, g.sx);
     const int ty0 = tiles_begin(c.gy), ty1 = HALO ? ty0 + 1 : tiles_end(c.gy, g.sy);
+    const unsigned rows = (pos_w != bool(degrid_pass)) ? rows_m : rows_mc;
     for (int gpol = 0; gpol < geo.nr; ++gpol) {
-      if (!row_has_index((pos_w != bool(degrid_pass)) ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
+      if (!((rows >> gpol) & 1u)) continue;
       const int base = (int(gcube) * geo.nr + gpol) * ntiles;
       for (int ty = ty0; ty < ty1; ++ty)
         for (int tx = tx0; tx < tx1; ++tx)
@@ -904,8 +939,9 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
     const int tx1 = HALO ? tx0 + 1 : tiles_end(e[j].h.gx, sxy[j][0]);
     const int ty0 = tiles_begin(e[j].h.gy);
     const int ty1 = HALO ? ty0 + 1 : tiles_end(e[j].h.gy, sxy[j][1]);
+    const unsigned rows = (pos_w != bool(degrid_pass)) ? rows_m : rows_mc;
     for (int gpol = 0; gpol < geo.nr; ++gpol) {
-      if (!row_has_index((pos_w != bool(degrid_pass)) ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
+      if (!((rows >> gpol) & 1u)) continue;
       const int base = (int(gc[j]) * geo.nr + gpol) * ntiles;
       for (int ty = ty0; ty < ty1; ++ty)
         for (int tx = tx0; tx < tx1; ++tx) {
@@ -2253,13 +2289,11 @@ __device__ __forceinline__ int sub_lane_y(int lane) {
   return (g >> 1) + 4 * (g & 1);
 }
 
-// a batch entry prepared by its lane
-template <int N>
-struct SubPrep {
-  unsigned w0;        // region pixel | sx << 12 | sy << 18 | use << 24 | flags
-  unsigned slice[N];  // first tap of each polarization's CF slice
-  float2 g[N];
-};
+// PK (one polarization, CF of fewer than 2^21 taps): an entry's region
+// pixel, w-sign and CF slice travel in one word, read once per slot
+constexpr unsigned sub_pk_flip = 1u << 10;
+constexpr int sub_pk_shift = 11;
+constexpr size_t sub_pk_taps = size_t(1) << (32 - sub_pk_shift);
 
 // exp(i phi) rounded to FP32 for an FP64 phase: phi is reduced to
 // [-pi, pi] in FP64 (2 pi split into a 53-bit head and its tail, exact for
@@ -2330,8 +2364,8 @@ __device__ __forceinline__ void sub_gradient_entry(
 // PX: region pixels per lane; each wave's region is padded to PX * 64
 // pixels (>= (16 + hx) * (16 + hy)), so a lane folds pixels lane + 64 k with
 // no bounds test.  UNI: every CF group has the same support, whose lane mask
-// then holds for the whole call.
-template <int N, int PX, bool UNI>
+// then holds for the whole call.  PK: see sub_pk_flip.
+template <int N, int PX, bool UNI, bool PK>
 __global__ __launch_bounds__(sub_block) __attribute__((amdgpu_waves_per_eu(N == 1 ? 8 : 4)))
 void k_sub(
   Geometry geo, const GroupInfo* __restrict__ groups, const float2* __restrict__ taps,
@@ -2439,6 +2473,9 @@ void k_sub(
     row_m[v] = mu.M[gpol][v];
     row_mc[v] = mu.Mc[gpol][v];
   }
+  // the quadrant's list, taken by its two waves alternately 64 entries at
+  // a time (fewer partly filled batches than an even split, whose per-batch
+  // costs outweigh its balance)
   const unsigned total = seg_pref[q][nbins];
   auto locate = [&](unsigned i) {
     int lo = 0, hi = nbins;  // seg_pref[q][lo] <= i < seg_pref[q][hi]
@@ -2483,7 +2520,8 @@ void k_sub(
       e = entries[ei];
     }
     // p_addr: the entry's LDS byte offset from the region origin, with the
-    // w-sign flip (bit 31: conj(c) for w > 0) -- 0 for slots that add nothing
+    // w-sign flip (bit 31: conj(c) for w > 0) -- 0 for slots that add
+    // nothing.  PK: the region pixel | sub_pk_flip | slice << sub_pk_shift
     unsigned p_addr = 0, p_key = 0;
     unsigned p_sl[N];
     float p_gx[N], p_gy[N];
@@ -2523,7 +2561,11 @@ void k_sub(
       const unsigned base = unsigned((ly - qy * quad_edge) * QX + (lx - qx * quad_edge));
       p_key = unsigned(g.sx) | (unsigned(g.sy) << 6);
       normal = !grad && use != 0u;
-      if (normal) p_addr = base * unsigned(sizeof(float2)) | (pos_w ? 0x80000000u : 0u);
+      if constexpr (PK) {
+        p_addr = (p_sl[0] << sub_pk_shift) | (normal ? base | (pos_w ? sub_pk_flip : 0u) : 0u);
+      } else {
+        if (normal) p_addr = base * unsigned(sizeof(float2)) | (pos_w ? 0x80000000u : 0u);
+      }
       grad = grad && use != 0u;
     }
     const unsigned long long nmask = __ballot(normal);
@@ -2536,6 +2578,7 @@ void k_sub(
     // from it while it is active); the slot's other values are read from
     // its lane when it is added
     float2 ring[sub_ring][N];
+    unsigned ring_pk[sub_ring];               // PK: the slot's word
     unsigned long long ring_lanes[sub_ring];  // !UNI: lanes of the slot's support
     auto fetch = [&](int j) {
       const int r = j % sub_ring;
@@ -2547,19 +2590,34 @@ void k_sub(
         }
         ring_lanes[r] = tap_lanes;
       }
-#pragma unroll
-      for (int v = 0; v < N; ++v) {
-        const int slv = __builtin_amdgcn_readlane(int(p_sl[v]), j);
+      if constexpr (PK) {
+        const unsigned pk = unsigned(__builtin_amdgcn_readlane(int(p_addr), j));
+        ring_pk[r] = pk;
         const v2i_t raw = __builtin_amdgcn_raw_buffer_load_b64(
-          tap_rsrc, int(tap_off), slv * int(sizeof(float2)), 0);
-        ring[r][v] = make_float2(__int_as_float(raw.x), __int_as_float(raw.y));
+          tap_rsrc, int(tap_off), int((pk >> sub_pk_shift) * unsigned(sizeof(float2))), 0);
+        ring[r][0] = make_float2(__int_as_float(raw.x), __int_as_float(raw.y));
+      } else {
+#pragma unroll
+        for (int v = 0; v < N; ++v) {
+          const int slv = __builtin_amdgcn_readlane(int(p_sl[v]), j);
+          const v2i_t raw = __builtin_amdgcn_raw_buffer_load_b64(
+            tap_rsrc, int(tap_off), slv * int(sizeof(float2)), 0);
+          ring[r][v] = make_float2(__int_as_float(raw.x), __int_as_float(raw.y));
+        }
       }
     };
     auto add = [&](int j) {
       const int r = j % sub_ring;
-      const unsigned aw = unsigned(__builtin_amdgcn_readlane(int(p_addr), j));
-      const unsigned flip = aw & 0x80000000u;
-      const unsigned addr = aw & 0x7fffffffu;
+      unsigned flip, addr;
+      if constexpr (PK) {
+        const unsigned pk = ring_pk[r];
+        flip = (pk << (31 - 10)) & 0x80000000u;
+        addr = (pk & (sub_pk_flip - 1u)) * unsigned(sizeof(float2));
+      } else {
+        const unsigned aw = unsigned(__builtin_amdgcn_readlane(int(p_addr), j));
+        flip = aw & 0x80000000u;
+        addr = aw & 0x7fffffffu;
+      }
       bool on;
       if constexpr (UNI) on = uni_ok; else on = __builtin_amdgcn_inverse_ballot_w64(ring_lanes[r]);
       float h_r = 0.0f, h_i = 0.0f;
@@ -2571,10 +2629,11 @@ void k_sub(
         const float gx = __uint_as_float(ux), gy = __uint_as_float(uy);
         // conj(c) g for w > 0: c.y g.y and c.y g.x enter with the sign flipped
         const float gcx = __uint_as_float(uy ^ flip), gcy = __uint_as_float(ux ^ flip);
-        // each product rounded as the reference's FP32 complex product
+        // the FP32 complex product, each part as one product and one fused
+        // multiply-add
         const float2 c = ring[r][v];
-        const float pr = c.x * gx - c.y * gcx;
-        const float pi = c.x * gy + c.y * gcy;
+        const float pr = __builtin_fmaf(c.x, gx, -(c.y * gcx));
+        const float pi = __builtin_fmaf(c.x, gy, c.y * gcy);
         if constexpr (N == 1) {
           h_r = pr;
           h_i = pi;
@@ -2632,7 +2691,8 @@ void k_sub(
         for (int v = 0; v < N; ++v) {
           const int m = gpos_w ? row_m[v] : row_mc[v];
           guse |= (m >= 0 && m < gg.nmueller) ? 1u << v : 0u;
-          sl[v] = unsigned(__builtin_amdgcn_readlane(int(p_sl[v]), j));
+          sl[v] = PK ? unsigned(__builtin_amdgcn_readlane(int(p_addr), j)) >> sub_pk_shift
+                     : unsigned(__builtin_amdgcn_readlane(int(p_sl[v]), j));
           gv[v] = ge.g[v];
         }
         const unsigned gbase = ((gpos >> 6) & 63u) - unsigned(qy * quad_edge);
@@ -3528,27 +3588,33 @@ hipError_t launch_tile_sub(
   const int px = (Q + wave - 1) / wave;  // 4 .. 9
   const int pxc = px <= 4 ? 4 : px <= 6 ? 6 : px <= 8 ? 8 : 9;
   const size_t dyn = size_t(sub_waves) * pxc * wave * sizeof(float2);
-#define HPGX_SUB(NP, PX)                                                       \
+#define HPGX_SUB(NP, PX, PK)                                                   \
   hipLaunchKernelGGL(                                                          \
-    uni ? (k_sub<NP, PX, true>) : (k_sub<NP, PX, false>),                      \
+    uni ? (k_sub<NP, PX, true, PK>) : (k_sub<NP, PX, false, PK>),              \
     dim3(unsigned(ntile_keys)), dim3(sub_block), dyn, s,                       \
     geo, cf.groups, cf.taps, unsigned(taps_bytes), mu, cb.off_table, nb,       \
     block_cap,                                                                 \
     static_cast<const Entry4<NP>*>(cb.stream_entries), cb.entry_vis,           \
     static_cast<const VisRec<NP>*>(cb.vis), hx, hy, static_cast<float2*>(grid),\
     cb.halo, cb.halo_set)
-#define HPGX_SUB_PX(NP)                                                        \
+#define HPGX_SUB_PX(NP, PK)                                                    \
   switch (pxc) {                                                               \
-  case 4: HPGX_SUB(NP, 4); break;                                              \
-  case 6: HPGX_SUB(NP, 6); break;                                              \
-  case 8: HPGX_SUB(NP, 8); break;                                              \
-  default: HPGX_SUB(NP, 9); break;                                             \
+  case 4: HPGX_SUB(NP, 4, PK); break;                                          \
+  case 6: HPGX_SUB(NP, 6, PK); break;                                          \
+  case 8: HPGX_SUB(NP, 8, PK); break;                                          \
+  default: HPGX_SUB(NP, 9, PK); break;                                         \
   }
+  // one word per slot: region pixels below sub_pk_flip, slices addressed
+  // by tap index below sub_pk_taps
+  static_assert(9 * wave < int(sub_pk_flip), "region pixel in the packed word");
+  const bool pk = npol == 1 && taps_bytes / sizeof(float2) < sub_pk_taps;
   switch (npol) {
-  case 1: HPGX_SUB_PX(1); break;
-  case 2: HPGX_SUB_PX(2); break;
-  case 3: HPGX_SUB_PX(3); break;
-  case 4: HPGX_SUB_PX(4); break;
+  case 1:
+    if (pk) HPGX_SUB_PX(1, true) else HPGX_SUB_PX(1, false)
+    break;
+  case 2: HPGX_SUB_PX(2, false); break;
+  case 3: HPGX_SUB_PX(3, false); break;
+  case 4: HPGX_SUB_PX(4, false); break;
   default: return hipErrorInvalidValue;
   }
 #undef HPGX_SUB_PX
