@@ -2266,14 +2266,17 @@ __global__ __launch_bounds__(quad_block) void k_quad(
 // batch's entries one at a time, one lane per CF tap, to its own FP32
 // region of (16 + hx) x (16 + hy) pixels in LDS by a plain read-modify-write
 // (an entry's taps are distinct pixels, the region is the wave's own and a
-// wave's LDS operations execute in order).  After every sub_fold entries
-// the wave folds the region into registers: lane l holds the (hi, lo)
-// pairs of region pixels l, l + 64, ..., adds each pixel's FP32 sum to its
-// pair by two-sum and clears the pixel.  So the products of a pixel are
-// summed in FP32 over at most sub_fold entries, and those sums are
-// accumulated with compensation.  The flush combines the pairs of a
-// quadrant's two waves, then the four quadrants over the tile and halo
-// (quad_flush_tile), as k_quad does, so k_halo folds the halos unchanged.
+// wave's LDS operations execute in order).  The batches run in rounds,
+// in step across the workgroup, each of two halves of 32 slots; after each
+// half every wave folds (between two barriers): wave q + 4 h holds the
+// (hi, lo) pairs of half h of quadrant q's region pixels (lane l: pixels
+// l + 64 k), adds both of the quadrant's regions' FP32 sums of those pixels
+// to its pairs by two-sum and clears them.  So the products of a pixel are
+// summed in FP32 over at most sub_fold entries of a region, those sums are
+// accumulated with compensation, and a wave keeps the pairs of half a
+// region in registers.  The flush sums the four quadrants' pairs over the
+// tile and halo (quad_flush_tile), as k_quad does, so k_halo folds the
+// halos unchanged.
 constexpr int sub_waves = 8;
 constexpr int sub_block = sub_waves * wave;
 constexpr int sub_bins_max = 256;  // bin blocks per pass (LDS segment tables)
@@ -2444,21 +2447,39 @@ void k_sub(
   float2* region = sregions + size_t(w) * QP;
 #pragma unroll
   for (int k = 0; k < PX; ++k) region[lane + k * wave] = make_float2(0.0f, 0.0f);
-  float4 acc[PX];
+  // the (hi, lo) pairs of this wave's share of its quadrant's pixels: wave
+  // half 0 holds pixels lane + 64 k for k < PXA, half 1 the rest
+  constexpr int PXA = (PX + 1) / 2;
+  const int k0 = half ? PXA : 0;
+  const int nk = half ? PX - PXA : PXA;
+  float4 acc[PXA];
 #pragma unroll
-  for (int k = 0; k < PX; ++k) acc[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  // the FP32 sums of the region into the (hi, lo) pairs, by two-sum
+  for (int k = 0; k < PXA; ++k) acc[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float2* reg_a = sregions + size_t(q) * QP;      // the quadrant's two regions
+  const float2* reg_b = sregions + size_t(q + 4) * QP;
+  // the FP32 sums of both regions' pixels of this wave's share into its
+  // (hi, lo) pairs, by two-sum, clearing the pixels (all waves at once,
+  // between barriers)
   auto fold = [&]() {
 #pragma unroll
-    for (int k = 0; k < PX; ++k) {
-      const int px = lane + k * wave;
-      const float2 p = region[px];
-      region[px] = make_float2(0.0f, 0.0f);
+    for (int k = 0; k < PXA; ++k) {
+      if (PX % 2 == 1 && k >= nk) continue;
+      const int px = lane + (k0 + k) * wave;
+      const float2 pa = reg_a[px];
+      const float2 pb = reg_b[px];
+      const_cast<float2*>(reg_a)[px] = make_float2(0.0f, 0.0f);
+      const_cast<float2*>(reg_b)[px] = make_float2(0.0f, 0.0f);
       float sum, err;
-      two_sum(acc[k].x, p.x, sum, err);
+      two_sum(acc[k].x, pa.x, sum, err);
       acc[k].x = sum;
       acc[k].z += err;
-      two_sum(acc[k].y, p.y, sum, err);
+      two_sum(acc[k].y, pa.y, sum, err);
+      acc[k].y = sum;
+      acc[k].w += err;
+      two_sum(acc[k].x, pb.x, sum, err);
+      acc[k].x = sum;
+      acc[k].z += err;
+      two_sum(acc[k].y, pb.y, sum, err);
       acc[k].y = sum;
       acc[k].w += err;
     }
@@ -2507,7 +2528,23 @@ void k_sub(
   int tap_key = -1;
   unsigned long long tap_lanes = 0ull;
 
-  for (unsigned b0 = unsigned(half) * wave; b0 < total; b0 += 2 * wave) {
+  // rounds: in round r, wave half h takes batch 2 r + h of its quadrant's
+  // list; all eight waves run every round (the folds are shared)
+  const unsigned tq[4] = {seg_pref[0][nbins], seg_pref[1][nbins], seg_pref[2][nbins],
+                          seg_pref[3][nbins]};
+  const unsigned tmax = max(max(tq[0], tq[1]), max(tq[2], tq[3]));
+  const unsigned nrounds = (tmax + 2u * wave - 1u) / (2u * wave);
+  for (unsigned rd = 0; rd < nrounds; ++rd) {
+    const unsigned b0 = (2u * rd + unsigned(half)) * wave;
+    // entries in the round's largest batch over the workgroup (uniform)
+    unsigned rmax = 0;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+      for (unsigned hh = 0; hh < 2; ++hh) {
+        const unsigned bb = (2u * rd + hh) * wave;
+        rmax = max(rmax, tq[qq] > bb ? min(unsigned(wave), tq[qq] - bb) : 0u);
+      }
     // lane j: entry b0 + j of the quadrant's list, prepared.  Slots that
     // add nothing here (past the list, phase gradient, no CF index) keep
     // zero values and the region's first pixel, and read a valid CF slice.
@@ -2655,10 +2692,13 @@ void k_sub(
         *a = tv;
       }
     };
-    // two halves of 32 slots, each followed by a fold (so at most 32
-    // entries between folds)
+    // two halves of 32 slots (then the half's entries with a phase
+    // gradient), each followed by the shared fold, so at most 32 entries
+    // reach a region between folds; a half without entries in any wave of
+    // the workgroup is skipped
 #pragma unroll
     for (int hb = 0; hb < 2; ++hb) {
+      if (unsigned(hb * 32) >= rmax) continue;
       if (hb * 32 < nslots) {
 #pragma unroll
         for (int j = 0; j < sub_ring; ++j) fetch(hb * 32 + j);
@@ -2667,13 +2707,8 @@ void k_sub(
           add(hb * 32 + j);
           if (j + sub_ring < 32) fetch(hb * 32 + j + sub_ring);
         }
-        fold();
       }
-    }
-    // entries with a phase gradient, after the batch's others
-    unsigned long long gl = gtodo;
-    if (gl) {
-      int since = 0;
+      unsigned long long gl = gtodo & (hb ? 0xffffffff00000000ull : 0xffffffffull);
       while (gl) {
         const int j = __builtin_ctzll(gl);
         gl &= gl - 1ull;
@@ -2700,41 +2735,20 @@ void k_sub(
           | (unsigned(gg.sx) << 12) | (unsigned(gg.sy) << 18) | (guse << 24)
           | (gpos_w ? quad_pos_w : 0u);
         sub_gradient_entry<N>(w0, sl, gv, entry_vis[ej], ge.h.pos, geo, taps, recs, QX, region);
-        if (++since >= sub_fold) {
-          fold();
-          since = 0;
-        }
       }
+      __syncthreads();
       fold();
+      __syncthreads();
     }
   }
-  __syncthreads();
 
-  // combine the pairs of the quadrant's two waves, then flush
+  // each wave's pairs into LDS (the regions are no longer read), then flush
   float4* pairs = reinterpret_cast<float4*>(sregions);
-  if (half == 1) {
 #pragma unroll
-    for (int k = 0; k < PX; ++k)
-      if (lane + k * wave < Q) pairs[q * Q + lane + k * wave] = acc[k];
-  }
-  __syncthreads();
-  if (half == 0) {
-#pragma unroll
-    for (int k = 0; k < PX; ++k) {
-      const int px = lane + k * wave;
-      if (px < Q) {
-        const float4 b = pairs[q * Q + px];
-        float sum, err;
-        float4 r;
-        two_sum(acc[k].x, b.x, sum, err);
-        r.x = sum;
-        r.z = acc[k].z + (err + b.z);
-        two_sum(acc[k].y, b.y, sum, err);
-        r.y = sum;
-        r.w = acc[k].w + (err + b.w);
-        pairs[q * Q + px] = r;
-      }
-    }
+  for (int k = 0; k < PXA; ++k) {
+    if (PX % 2 == 1 && k >= nk) continue;
+    const int px = lane + (k0 + k) * wave;
+    if (px < Q) pairs[q * Q + px] = acc[k];
   }
   __syncthreads();
   quad_flush_tile<sub_block, sub_flush>(pairs, QX, QY, hx, hy, geo, plane, x0, y0, grid,
