@@ -2267,8 +2267,9 @@ __global__ __launch_bounds__(quad_block) void k_quad(
 // region of (16 + hx) x (16 + hy) pixels in LDS by a plain read-modify-write
 // (an entry's taps are distinct pixels, the region is the wave's own and a
 // wave's LDS operations execute in order).  The batches run in rounds,
-// in step across the workgroup, each of two halves of 32 slots; after each
-// half every wave folds (between two barriers): wave q + 4 h holds the
+// in step across the workgroup (two halves of 32 slots, each followed by
+// its entries with a phase gradient); after each round every wave folds
+// (between two barriers): wave q + 4 h holds the
 // (hi, lo) pairs of half h of quadrant q's region pixels (lane l: pixels
 // l + 64 k), adds both of the quadrant's regions' FP32 sums of those pixels
 // to its pairs by two-sum and clears them.  So the products of a pixel are
@@ -2280,7 +2281,7 @@ __global__ __launch_bounds__(quad_block) void k_quad(
 constexpr int sub_waves = 8;
 constexpr int sub_block = sub_waves * wave;
 constexpr int sub_bins_max = 256;  // bin blocks per pass (LDS segment tables)
-constexpr int sub_fold = 32;       // entries between folds
+constexpr int sub_fold = 64;       // entries of a region between folds (one batch)
 constexpr int sub_ring = 8;        // slots whose CF taps load ahead
 constexpr int sub_flush = 3;       // tile + halo pixels per thread in the flush
 
@@ -2692,10 +2693,10 @@ void k_sub(
         *a = tv;
       }
     };
-    // two halves of 32 slots (then the half's entries with a phase
-    // gradient), each followed by the shared fold, so at most 32 entries
-    // reach a region between folds; a half without entries in any wave of
-    // the workgroup is skipped
+    // two halves of 32 slots, each followed by the half's entries with a
+    // phase gradient; the shared fold after the round's last half, so at
+    // most sub_fold entries reach a region between folds.  A half without
+    // entries in any wave of the workgroup is skipped
 #pragma unroll
     for (int hb = 0; hb < 2; ++hb) {
       if (unsigned(hb * 32) >= rmax) continue;
@@ -2709,6 +2710,7 @@ void k_sub(
         }
       }
       unsigned long long gl = gtodo & (hb ? 0xffffffff00000000ull : 0xffffffffull);
+      const bool last_half = hb == 1 || 32u >= rmax;
       while (gl) {
         const int j = __builtin_ctzll(gl);
         gl &= gl - 1ull;
@@ -2736,9 +2738,11 @@ void k_sub(
           | (gpos_w ? quad_pos_w : 0u);
         sub_gradient_entry<N>(w0, sl, gv, entry_vis[ej], ge.h.pos, geo, taps, recs, QX, region);
       }
-      __syncthreads();
-      fold();
-      __syncthreads();
+      if (last_half) {
+        __syncthreads();
+        fold();
+        __syncthreads();
+      }
     }
   }
 
