@@ -136,6 +136,15 @@ struct hpgx_state_s {
   // haloed bins: quadrant-keyed bins and k_sub instead of k_quad
   // (HPGX_SUB_TILES=0 keeps k_quad)
   bool sub_tiles = true;
+  // the finishing chain, deferred until an entry point touches the grid
+  // (flush_finishing): normalize_by_weights, an in-place grid FFT and, for
+  // even grid sizes, a shift directly after that FFT
+  bool pend_norm = false;
+  double pend_wfactor = 1.0;
+  bool pend_fft = false;
+  double pend_fft_norm = 1.0;
+  int pend_fft_sign = 0;
+  bool pend_shift = false;
 
   Plan plan_z2z, plan_c2c;
   hipEvent_t events[num_event_slots] = {};
@@ -586,6 +595,37 @@ int fft_planes(hpgx_state_t st, void* data, bool fp64, double norm, int sign,
                    st->stream, fp64 ? 0 : 1, out, 1.0 / norm, st->grid_elems()));
   }
   *result = out;
+  return HPGX_SUCCESS;
+}
+
+// Runs the deferred finishing chain.  With an FFT pending, normalize (the
+// FP64 division by the weights), the FFT's 1 / norm and, with a shift
+// pending, the (-1)^(x + y) modulation that stands for the shift of the
+// FFT's output (even sizes: the FFT of x (-1)^(x+y) is the FFT of x shifted
+// by half the grid on both axes) are one pass before the FFT
+// (src/hpg_core.hpp:1849-1860, 1863-1889, 2329-2355); without one,
+// normalize runs alone.
+int flush_finishing(hpgx_state_t st) {
+  if (!st->pend_norm && !st->pend_fft && !st->pend_shift) return HPGX_SUCCESS;
+  const bool norm = st->pend_norm, fft = st->pend_fft, shift = st->pend_shift;
+  st->pend_norm = st->pend_fft = st->pend_shift = false;
+  const int mode = st->mode == HPGX_GRID_FP64 ? 0 : 1;
+  if (!fft) {
+    HPGX_HIP(st, hpgx::launch_normalize(st->stream, mode, st->grid, st->weights,
+                                        st->pend_wfactor, st->geometry()));
+    return HPGX_SUCCESS;
+  }
+  const bool pre = norm || shift;
+  if (pre) {
+    HPGX_HIP(st, hpgx::launch_finish_pre(
+                   st->stream, mode, st->grid, norm ? st->weights : nullptr, st->pend_wfactor,
+                   1.0 / st->pend_fft_norm, shift, st->geometry()));
+  }
+  void* out = nullptr;
+  if (int rc = fft_planes(st, st->grid, mode == 0, pre ? 1.0 : st->pend_fft_norm,
+                          st->pend_fft_sign, true, &out))
+    return rc;
+  st->grid = out;
   return HPGX_SUCCESS;
 }
 
@@ -1042,6 +1082,7 @@ int hpgx_clone(hpgx_state_t src, hpgx_state_t* dst) {
   if (!src || !dst) return HPGX_ERR_OTHER;
   *dst = nullptr;
   if (int rc = set_device(src)) return rc;
+  if (int rc = flush_finishing(src)) return rc;
   HPGX_HIP(src, hipStreamSynchronize(src->stream));
   hpgx_state_t st = nullptr;
   std::vector<unsigned> ext;
@@ -1294,6 +1335,7 @@ int hpgx_grid_visibilities(
   if (return_visibilities && !vis_out)
     return fail(st, HPGX_ERR_OTHER, "null visibilities_out");
   if (int rc = set_device(st)) return rc;
+  if (int rc = flush_finishing(st)) return rc;
   if (st->cf.ngroups == 0) {
     // no CF: every record refers to a missing CF group and is skipped
     if (return_visibilities && vis_out != vis) {
@@ -1353,6 +1395,7 @@ int hpgx_grid_visibilities_noalloc(
   if (n == 0) return HPGX_SUCCESS;
   if (!vis) return fail(st, HPGX_ERR_OTHER, "null visibilities");
   if (int rc = set_device(st)) return rc;
+  if (int rc = flush_finishing(st)) return rc;
   // no CF: every record refers to a missing CF group and is skipped
   if (st->cf.ngroups == 0) return HPGX_SUCCESS;
   void* base = nullptr;
@@ -1377,6 +1420,7 @@ int hpgx_degrid_adjoint(
   if (n == 0 || st->cf.ngroups == 0) return HPGX_SUCCESS;
   if (!vis || !residuals) return fail(st, HPGX_ERR_OTHER, "null argument");
   if (int rc = set_device(st)) return rc;
+  if (int rc = flush_finishing(st)) return rc;
   if (int rc = ensure_call_buffers(st)) return rc;
   const bool in_place = memory == HPGX_DEVICE;
   if (!in_place)
@@ -1412,6 +1456,7 @@ int hpgx_degrid_forward(
   if (n == 0) return HPGX_SUCCESS;
   if (!vis || !predictions) return fail(st, HPGX_ERR_OTHER, "null argument");
   if (int rc = set_device(st)) return rc;
+  if (int rc = flush_finishing(st)) return rc;
   const size_t bytes = n * npol * sizeof(double2);
   double2* out = static_cast<double2*>(predictions);
   double2* tmp = nullptr;
@@ -1447,6 +1492,7 @@ int hpgx_degrid_forward(
 int hpgx_fence(hpgx_state_t st) {
   if (!st) return HPGX_ERR_OTHER;
   if (int rc = set_device(st)) return rc;
+  if (int rc = flush_finishing(st)) return rc;
   HPGX_HIP(st, hipStreamSynchronize(st->stream));
   for (hipStream_t cs : st->copy_streams)
     if (cs) HPGX_HIP(st, hipStreamSynchronize(cs));
@@ -1466,15 +1512,29 @@ int hpgx_fence(hpgx_state_t st) {
 int hpgx_normalize_by_weights(hpgx_state_t st, double wfactor) {
   if (!st) return HPGX_ERR_OTHER;
   if (int rc = set_device(st)) return rc;
-  HPGX_HIP(st, hpgx::launch_normalize(
-                 st->stream, st->mode == HPGX_GRID_FP64 ? 0 : 1, st->grid, st->weights,
-                 wfactor, st->geometry()));
+  // deferred: it joins the pass before a following FFT
+  if (st->pend_norm || st->pend_fft)
+    if (int rc = flush_finishing(st)) return rc;
+  st->pend_norm = true;
+  st->pend_wfactor = wfactor;
   return HPGX_SUCCESS;
 }
 
 int hpgx_apply_grid_fft(hpgx_state_t st, double norm, int sign, int in_place) {
   if (!st) return HPGX_ERR_OTHER;
   if (int rc = set_device(st)) return rc;
+  if (in_place) {
+    // deferred (its plan made now, so that a plan failure is reported here)
+    if (st->pend_fft)
+      if (int rc = flush_finishing(st)) return rc;
+    hipfftHandle h;
+    if (int rc = get_plan(st, st->mode == HPGX_GRID_FP64, h)) return rc;
+    st->pend_fft = true;
+    st->pend_fft_norm = norm;
+    st->pend_fft_sign = sign;
+    return HPGX_SUCCESS;
+  }
+  if (int rc = flush_finishing(st)) return rc;
   void* out = nullptr;
   if (int rc = fft_planes(st, st->grid, st->mode == HPGX_GRID_FP64, norm, sign,
                           in_place != 0, &out))
@@ -1496,6 +1556,13 @@ int hpgx_apply_model_fft(hpgx_state_t st, double norm, int sign, int in_place) {
 int hpgx_shift_grid(hpgx_state_t st, int direction) {
   if (!st) return HPGX_ERR_OTHER;
   if (int rc = set_device(st)) return rc;
+  // directly after a deferred FFT of an even grid the shift is the FFT
+  // input's modulation (both directions shift by half the grid)
+  if (st->pend_fft && !st->pend_shift && st->gsize[0] % 2 == 0 && st->gsize[1] % 2 == 0) {
+    st->pend_shift = true;
+    return HPGX_SUCCESS;
+  }
+  if (int rc = flush_finishing(st)) return rc;
   if (int rc = ensure_scratch(st)) return rc;
   HPGX_HIP(st, hpgx::launch_shift(
                  st->stream, st->mode == HPGX_GRID_FP64 ? 0 : 1, st->grid, st->scratch,
@@ -1523,6 +1590,8 @@ int hpgx_shift_model(hpgx_state_t st, int direction) {
 int hpgx_reset_grid(hpgx_state_t st) {
   if (!st) return HPGX_ERR_OTHER;
   if (int rc = set_device(st)) return rc;
+  // the deferred finishing chain of a grid that is cleared has no effect
+  st->pend_norm = st->pend_fft = st->pend_shift = false;
   HPGX_HIP(st, hipStreamSynchronize(st->stream));
   if (int rc = zero_grid(st)) return rc;
   HPGX_HIP(st, hipStreamSynchronize(st->stream));
@@ -1532,6 +1601,7 @@ int hpgx_reset_grid(hpgx_state_t st) {
 int hpgx_grid_values(hpgx_state_t st, void* dst, int layout) {
   if (!st || !dst) return HPGX_ERR_OTHER;
   if (int rc = set_device(st)) return rc;
+  if (int rc = flush_finishing(st)) return rc;
   HPGX_HIP(st, hipStreamSynchronize(st->stream));
   return copy_grid_to_host(st, st->grid, st->mode != HPGX_GRID_FP64, dst, layout);
 }
@@ -1573,6 +1643,7 @@ int hpgx_grid_window(
       || mrow >= st->gsize[2] || cube >= st->gsize[3])
     return fail(st, HPGX_ERR_OTHER, "grid window out of bounds");
   if (int rc = set_device(st)) return rc;
+  if (int rc = flush_finishing(st)) return rc;
   HPGX_HIP(st, hipStreamSynchronize(st->stream));
   const size_t plane = size_t(cube) * st->gsize[2] + mrow;
   const size_t ps = size_t(st->gsize[0]) * st->gsize[1];
@@ -1596,6 +1667,8 @@ int hpgx_grid_window(
 
 int hpgx_grid_device_ptr(hpgx_state_t st, void** ptr, size_t* count) {
   if (!st) return HPGX_ERR_OTHER;
+  if (int rc = set_device(st)) return rc;
+  if (int rc = flush_finishing(st)) return rc;
   if (ptr) *ptr = st->grid;
   if (count) *count = st->grid_elems();
   return HPGX_SUCCESS;
@@ -1604,6 +1677,7 @@ int hpgx_grid_device_ptr(hpgx_state_t st, void** ptr, size_t* count) {
 int hpgx_event_record(hpgx_state_t st, int slot) {
   if (!st || slot < 0 || slot >= num_event_slots) return HPGX_ERR_OTHER;
   if (int rc = set_device(st)) return rc;
+  if (int rc = flush_finishing(st)) return rc;
   if (!st->events[slot]) HPGX_HIP(st, hipEventCreate(&st->events[slot]));
   HPGX_HIP(st, hipEventRecord(st->events[slot], st->stream));
   return HPGX_SUCCESS;

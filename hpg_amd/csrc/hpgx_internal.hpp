@@ -196,6 +196,12 @@ hipError_t launch_normalize(
   const Geometry& geo);
 hipError_t launch_scale(
   hipStream_t s, int mode, void* grid, double inv_norm, size_t count);
+// the deferred finishing chain's pass before its FFT: divided by wfactor *
+// the plane's weight (weights not null), times scale, and with modulate
+// multiplied by (-1)^(x + y)
+hipError_t launch_finish_pre(
+  hipStream_t s, int mode, void* grid, const double* weights, double wfactor, double scale,
+  bool modulate, const Geometry& geo);
 hipError_t launch_shift(
   hipStream_t s, int mode, void* grid, void* scratch, int direction,
   const Geometry& geo);

@@ -3288,6 +3288,36 @@ __global__ void k_normalize(
   grid[k] = g;
 }
 
+// the finishing chain's one elementwise pass before its FFT: each value
+// divided by wfactor * its plane's weight (when weights is given: the
+// GridNormalizer's FP64 division, src/hpg_core.hpp:1849-1860), times s,
+// and negated where x + y is odd (modulate: for even nx, ny the FFT of
+// x (-1)^(x+y) is the FFT of x shifted by half the grid on both axes, so
+// the shift that follows the FFT needs no pass of its own)
+template <typename T>
+__global__ void k_finish_pre(
+  T* __restrict__ grid, const double* __restrict__ weights, double wfactor, double s,
+  int modulate, int nx, size_t plane_size, size_t count) {
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  T g = grid[k];
+  double vx = double(g.x), vy = double(g.y);
+  if (weights) {
+    const double d = wfactor * weights[k / plane_size];
+    vx /= d;
+    vy /= d;
+  }
+  double f = s;
+  if (modulate) {
+    const size_t r = k % plane_size;
+    const size_t x = r % size_t(nx), y = r / size_t(nx);
+    if ((x + y) & 1u) f = -f;
+  }
+  g.x = vx * f;
+  g.y = vy * f;
+  grid[k] = g;
+}
+
 template <typename T>
 __global__ void k_scale(T* __restrict__ grid, double s, size_t count) {
   const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -3849,6 +3879,23 @@ hipError_t launch_normalize(
     hipLaunchKernelGGL(
       k_normalize<float2>, dim3(blocks_for(count)), dim3(block), 0, s,
       static_cast<float2*>(grid), weights, wfactor, ps, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_finish_pre(
+  hipStream_t s, int mode, void* grid, const double* weights, double wfactor, double scale,
+  bool modulate, const Geometry& geo) {
+  const size_t ps = size_t(geo.nx) * geo.ny;
+  const size_t count = ps * geo.nr * geo.nc;
+  if (count == 0) return hipSuccess;
+  if (mode == 0)
+    hipLaunchKernelGGL(
+      k_finish_pre<double2>, dim3(blocks_for(count)), dim3(block), 0, s,
+      static_cast<double2*>(grid), weights, wfactor, scale, int(modulate), geo.nx, ps, count);
+  else
+    hipLaunchKernelGGL(
+      k_finish_pre<float2>, dim3(blocks_for(count)), dim3(block), 0, s,
+      static_cast<float2*>(grid), weights, wfactor, scale, int(modulate), geo.nx, ps, count);
   return hipGetLastError();
 }
 

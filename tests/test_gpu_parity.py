@@ -487,6 +487,60 @@ def test_finishing_passes_vs_oracle(hpg, oracle_mod, mode):
         assert e3 <= (1e-12 if mode == 0 else 2e-6), (d, e3)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("shape", [(64, 48), (63, 48), (64, 47)])
+def test_finishing_chain_vs_oracle(hpg, oracle_mod, mode, shape):
+    """normalize -> FFT -> shift with no read in between (one pass and the
+    FFT for even sizes, the shift a (-1)^(x+y) modulation before the FFT;
+    the separate passes otherwise), then gridding onto the finished grid,
+    another chain in the other order (FFT -> normalize), and a cleared
+    chain; each against the oracle running the same calls."""
+    from hpg_amd import workloads
+    npol, _, M, Mc, radii, nm, nc, os_ = CASES[1]
+    gsz = (shape[0], shape[1], 2, 2)
+    cf = workloads.random_cf(os_, radii, nmueller=nm, ncube=nc, seed=14)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), M, cf, mode, Mc)
+    tol = 1e-12 if mode == 0 else 2e-6
+    rec = workloads.random_visibilities(2000, npol, gsz, (1.0, 1.0), cf, seed=15)
+    gs.grid_visibilities(hpg.Device.Serial, rec)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    gs.normalize_by_weights(2.0)
+    gs.apply_grid_fft(norm=3.0, sign=hpg.FFTSign.NEGATIVE)
+    gs.shift_grid(hpg.ShiftDirection.FORWARD)
+    o.normalize_by_weights(2.0)
+    o.apply_fft(norm=3.0, sign="NEGATIVE")
+    o.shift("FORWARD")
+    e1, _ = rel_errors(gs.grid_values(), o.grid_values())
+    assert e1 <= tol, e1
+    # a chain still pending when records are gridded runs first
+    rec2 = workloads.random_visibilities(500, npol, gsz, (1.0, 1.0), cf, seed=16)
+    gs.apply_grid_fft(norm=1.0, sign=hpg.FFTSign.POSITIVE)
+    gs.shift_grid(hpg.ShiftDirection.BACKWARD)
+    gs.grid_visibilities(hpg.Device.Serial, rec2)
+    o.apply_fft(norm=1.0, sign="POSITIVE")
+    o.shift("BACKWARD")
+    o.grid_visibilities(oracle_view(rec2, oracle_mod), True)
+    e2, _ = rel_errors(gs.grid_values(), o.grid_values())
+    assert e2 <= tol, e2
+    # FFT before normalize, then a shift: no fusion across the normalize
+    gs.apply_grid_fft(norm=2.0, sign=hpg.FFTSign.NEGATIVE)
+    gs.normalize_by_weights(0.5)
+    gs.shift_grid(hpg.ShiftDirection.FORWARD)
+    o.apply_fft(norm=2.0, sign="NEGATIVE")
+    o.normalize_by_weights(0.5)
+    o.shift("FORWARD")
+    gs.fence()
+    e3, _ = rel_errors(gs.grid_values(), o.grid_values())
+    assert e3 <= tol, e3
+    # a pending chain on a cleared grid leaves it zero
+    gs.normalize_by_weights(1.0)
+    gs.apply_grid_fft()
+    gs.shift_grid()
+    gs.reset_grid()
+    assert not np.any(gs.grid_values())
+
+
 @pytest.mark.parametrize("shape", [(20, 20), (19, 19), (18, 19), (19, 18)])
 def test_shift_cycle(hpg, shape):
     """ShiftCycle (GridderStateUnitTests.cc:1860-1924) through the model."""
