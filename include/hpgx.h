@@ -254,6 +254,16 @@ HPGX_API int hpgx_fence(hpgx_state_t state);
 
 /* ---- finishing passes ------------------------------------------------ */
 
+/* hpgx_normalize_by_weights, hpgx_apply_grid_fft with in_place and, for a
+ * grid of even x and y sizes, hpgx_shift_grid directly after such an FFT are
+ * queued on the state and run when the next entry point that reads or
+ * writes the grid is called (gridding, degridding, fence, readback, window,
+ * device pointer, event record, clone): a normalize and a shift around an
+ * FFT then form one elementwise pass before the FFT (the shift as the
+ * (-1)^(x+y) modulation of the FFT's input), with the results of the
+ * separate passes.  hpgx_reset_grid drops a queued chain.  An error of the
+ * queued work is returned by the entry point that runs it. */
+
 /* StateT::normalize_by_weights + GridNormalizer
  * (src/hpg_runtime.hpp:1293-1310, src/hpg_core.hpp:1832-1861) */
 HPGX_API int hpgx_normalize_by_weights(hpgx_state_t state, double wfactor);
