@@ -365,8 +365,9 @@ CAL_BYTES = 256 << 20
 
 
 def pmc_workload(hpg, dev_records, cf):
-    """The program profiled by the two counter runs: a 256 MiB device copy
-    (calibration), then 3 gridding calls into each grid type."""
+    """The program profiled by the counter runs: a 256 MiB device copy
+    (calibration), then 3 gridding calls into each grid type and 3
+    degridding calls of the batch."""
     import torch
     a = torch.ones(CAL_BYTES, dtype=torch.uint8, device="cuda")
     b = torch.empty_like(a)
@@ -382,6 +383,12 @@ def pmc_workload(hpg, dev_records, cf):
             gs.grid_visibilities(hpg.Device.Serial, dev_records)
         gs.fence()
         gs.release()
+    # and 3 degridding calls of the batch (records returned), as run_degrid
+    import numpy as np
+    rng = np.random.default_rng(7)
+    model = (rng.standard_normal((1, 1, GRID, GRID))
+             + 1j * rng.standard_normal((1, 1, GRID, GRID)))
+    run_degrid(hpg, dev_records, cf, model, 3, 0)
     print(json.dumps({"pmc_workload": "done", "calibration_bytes": CAL_BYTES}), flush=True)
 
 
@@ -430,7 +437,7 @@ def pmc_summary(fetch_dir, write_dir):
         if k.startswith("k_bin"):
             per_call["fp32c"] += b
             per_call["fp64"] += b
-        elif (k.startswith("k_quad<") or k.startswith("k_tile4<1, 1")
+        elif (k.startswith("k_sub<") or k.startswith("k_quad<") or k.startswith("k_tile4<1, 1")
               or k.startswith("k_halo<HIP_vector_type<float")):
             per_call["fp32c"] += b
         elif k.startswith("k_tile4<1, 0") or k.startswith("k_halo<HIP_vector_type<double"):

@@ -374,7 +374,10 @@ int ensure_call_buffers(hpgx_state_t st) {
     const size_t per_entry =
       (halo ? hpgx::halo_entry_bytes(int(st->npol)) : hpgx::entry_bytes(int(st->npol)))
       + sizeof(unsigned);
-    const size_t budget = std::max(entry_budget_bytes() / per_entry, epr);
+    // entries per pass: at least one record's, and below 2^32 (bin counts,
+    // offsets and entry indexes are 32-bit)
+    const size_t budget = std::min<size_t>(std::max(entry_budget_bytes() / per_entry, epr),
+                                           0xffffffffull);
     size_t need = std::min(st->vis_capacity * epr, budget);
     if (use_block_local(st)) {
       // whole bin blocks of bin_block_records records each
@@ -615,11 +618,29 @@ int flush_finishing(hpgx_state_t st) {
                                         st->pend_wfactor, st->geometry()));
     return HPGX_SUCCESS;
   }
+  const Geometry geo = st->geometry();
+  if (!hpgx::finish_pre_fits(geo)) {
+    // the separate passes
+    if (norm)
+      HPGX_HIP(st, hpgx::launch_normalize(st->stream, mode, st->grid, st->weights,
+                                          st->pend_wfactor, geo));
+    void* out = nullptr;
+    if (int rc = fft_planes(st, st->grid, mode == 0, st->pend_fft_norm, st->pend_fft_sign,
+                            true, &out))
+      return rc;
+    st->grid = out;
+    if (shift) {
+      if (int rc = ensure_scratch(st)) return rc;
+      HPGX_HIP(st, hpgx::launch_shift(st->stream, mode, st->grid, st->scratch,
+                                      HPGX_SHIFT_FORWARD, geo));
+    }
+    return HPGX_SUCCESS;
+  }
   const bool pre = norm || shift;
   if (pre) {
     HPGX_HIP(st, hpgx::launch_finish_pre(
                    st->stream, mode, st->grid, norm ? st->weights : nullptr, st->pend_wfactor,
-                   1.0 / st->pend_fft_norm, shift, st->geometry()));
+                   1.0 / st->pend_fft_norm, shift, geo));
   }
   void* out = nullptr;
   if (int rc = fft_planes(st, st->grid, mode == 0, pre ? 1.0 : st->pend_fft_norm,

@@ -198,7 +198,9 @@ hipError_t launch_scale(
   hipStream_t s, int mode, void* grid, double inv_norm, size_t count);
 // the deferred finishing chain's pass before its FFT: divided by wfactor *
 // the plane's weight (weights not null), times scale, and with modulate
-// multiplied by (-1)^(x + y)
+// multiplied by (-1)^(x + y); for grids with finish_pre_fits (rows and
+// planes within a launch's y and z extents)
+bool finish_pre_fits(const Geometry& geo);
 hipError_t launch_finish_pre(
   hipStream_t s, int mode, void* grid, const double* weights, double wfactor, double scale,
   bool modulate, const Geometry& geo);

@@ -3293,29 +3293,28 @@ __global__ void k_normalize(
 // GridNormalizer's FP64 division, src/hpg_core.hpp:1849-1860), times s,
 // and negated where x + y is odd (modulate: for even nx, ny the FFT of
 // x (-1)^(x+y) is the FFT of x shifted by half the grid on both axes, so
-// the shift that follows the FFT needs no pass of its own)
+// the shift that follows the FFT needs no pass of its own).  The launch
+// is (x blocks, row y, plane), so no index needs a division.
 template <typename T>
 __global__ void k_finish_pre(
   T* __restrict__ grid, const double* __restrict__ weights, double wfactor, double s,
-  int modulate, int nx, size_t plane_size, size_t count) {
-  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (k >= count) return;
-  T g = grid[k];
+  int modulate, int nx, int ny) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nx) return;
+  const int y = blockIdx.y;
+  const size_t plane = blockIdx.z;
+  T* p = grid + (plane * size_t(ny) + size_t(y)) * size_t(nx) + x;
+  T g = *p;
   double vx = double(g.x), vy = double(g.y);
   if (weights) {
-    const double d = wfactor * weights[k / plane_size];
+    const double d = wfactor * weights[plane];
     vx /= d;
     vy /= d;
   }
-  double f = s;
-  if (modulate) {
-    const size_t r = k % plane_size;
-    const size_t x = r % size_t(nx), y = r / size_t(nx);
-    if ((x + y) & 1u) f = -f;
-  }
+  const double f = modulate && ((x + y) & 1) ? -s : s;
   g.x = vx * f;
   g.y = vy * f;
-  grid[k] = g;
+  *p = g;
 }
 
 template <typename T>
@@ -3865,12 +3864,19 @@ hipError_t launch_degrid(
   return hipGetLastError();
 }
 
+bool finish_pre_fits(const Geometry& geo) {
+  return geo.ny <= 65535 && size_t(geo.nr) * geo.nc <= 65535;
+}
+
 hipError_t launch_normalize(
   hipStream_t s, int mode, void* grid, const double* weights, double wfactor,
   const Geometry& geo) {
   const size_t ps = size_t(geo.nx) * geo.ny;
   const size_t count = ps * geo.nr * geo.nc;
   if (count == 0) return hipSuccess;
+  // the same FP64 division, times 1
+  if (finish_pre_fits(geo))
+    return launch_finish_pre(s, mode, grid, weights, wfactor, 1.0, false, geo);
   if (mode == 0)
     hipLaunchKernelGGL(
       k_normalize<double2>, dim3(blocks_for(count)), dim3(block), 0, s,
@@ -3885,17 +3891,19 @@ hipError_t launch_normalize(
 hipError_t launch_finish_pre(
   hipStream_t s, int mode, void* grid, const double* weights, double wfactor, double scale,
   bool modulate, const Geometry& geo) {
-  const size_t ps = size_t(geo.nx) * geo.ny;
-  const size_t count = ps * geo.nr * geo.nc;
-  if (count == 0) return hipSuccess;
+  const size_t planes = size_t(geo.nr) * geo.nc;
+  if (size_t(geo.nx) * geo.ny * planes == 0) return hipSuccess;
+  // grid rows and planes index the launch's y and z
+  if (!finish_pre_fits(geo)) return hipErrorInvalidValue;
+  const dim3 grid_dim(unsigned((geo.nx + block - 1) / block), unsigned(geo.ny), unsigned(planes));
   if (mode == 0)
     hipLaunchKernelGGL(
-      k_finish_pre<double2>, dim3(blocks_for(count)), dim3(block), 0, s,
-      static_cast<double2*>(grid), weights, wfactor, scale, int(modulate), geo.nx, ps, count);
+      k_finish_pre<double2>, grid_dim, dim3(block), 0, s,
+      static_cast<double2*>(grid), weights, wfactor, scale, int(modulate), geo.nx, geo.ny);
   else
     hipLaunchKernelGGL(
-      k_finish_pre<float2>, dim3(blocks_for(count)), dim3(block), 0, s,
-      static_cast<float2*>(grid), weights, wfactor, scale, int(modulate), geo.nx, ps, count);
+      k_finish_pre<float2>, grid_dim, dim3(block), 0, s,
+      static_cast<float2*>(grid), weights, wfactor, scale, int(modulate), geo.nx, geo.ny);
   return hipGetLastError();
 }
 
