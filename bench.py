@@ -104,6 +104,40 @@ def run_gpu(hpg, mode, dev_records, cf, steps, warmup):
     return wall * 1e3 / steps, dev_ms, fin
 
 
+def run_grid_sizes(hpg, cf, steps, warmup):
+    """The single-pol workload's FP32-compensated gridding call on larger
+    grids (1e6 records, u, v uniform within +-0.45 of each extent): device
+    ms per call on the state's stream and the HBM fraction of its
+    algorithmic bytes, per grid edge."""
+    import numpy as np
+    import torch
+    from hpg_amd import workloads
+    out = {}
+    for grid in (2048, 4096):
+        rec = workloads.config1_visibilities(NUM_VIS, grid, seed=100 + grid)
+        dev = torch.from_numpy(rec.view(np.uint8).copy()).cuda()
+        torch.cuda.synchronize()
+        gs = hpg.GridderState.create(hpg.Device.Hip, 0, NUM_VIS, cf, (grid, grid, 1, 1),
+                                     (1.0, 1.0), [[0]],
+                                     grid_mode=hpg.GridMode.FP32_COMPENSATED)
+        gs.set_convolution_function(hpg.Device.Serial, cf)
+        for _ in range(warmup):
+            gs.grid_visibilities(hpg.Device.Serial, dev)
+        gs.fence()
+        gs.event_record(4)
+        for _ in range(steps):
+            gs.grid_visibilities(hpg.Device.Serial, dev)
+        gs.event_record(5)
+        gs.fence()
+        ms = gs.event_elapsed_ms(4, 5) / steps
+        nbytes = NUM_VIS * 88 + 387200 + 2 * grid * grid * 8
+        out[f"{grid}x{grid}"] = {"device_ms_per_call": ms, "vis_per_s": NUM_VIS / (ms * 1e-3),
+                                 "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        gs.release()
+        del dev
+    return out
+
+
 def run_degrid(hpg, dev_records, cf, model, steps, warmup):
     """Degridding of the same batch from a complex<double> model, as
     degrid_get_predicted_visibilities: the records come back with their
@@ -578,6 +612,7 @@ def main():
             os.environ["HPGX_FP32_TILE"] = prev
 
     e2e = {f"tasks_{t}": run_host_batches(hpg, records, cf, t, 10) for t in (0, 2)}
+    sizes = run_grid_sizes(hpg, cf, max(1, args.steps // 4), max(1, args.warmup // 4))
 
     rng = np.random.default_rng(7)
     model = (rng.standard_normal((1, 1, GRID, GRID))
@@ -631,6 +666,7 @@ def main():
                    "bytes_per_call": degrid_bytes(NUM_VIS),
                    "hbm_frac": degrid_bytes(NUM_VIS) / (degrid_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                    "kernel": "k_bin (also writes the returned batch) + k_degrid4"},
+        "fp32c_larger_grids": sizes,
         "end_to_end": {
             "what": "grid_visibilities from host memory, records uploaded each call "
                     "(10 calls back to back), FP32-compensated grid",
