@@ -308,7 +308,16 @@ bool use_sub(hpgx_state_t st) {
   return st->sub_tiles && use_quad(st) && use_halo(st)
     && hpgx::sub_supported(halo_x(st), halo_y(st))
     && st->taps_capacity * sizeof(float2) <= size_t(0xffffffffu)
-    && 4 * st->nkeys <= size_t(hpgx::bin_keys_limit());
+    && 4 * st->nkeys <= size_t(hpgx::bin_quad_keys_limit());
+}
+
+// k_sub on global bins (quadrant keys counted and emitted through global
+// counters): grids of more keys than the block-local histogram holds, one
+// polarization
+bool use_sub_global(hpgx_state_t st) {
+  return st->sub_tiles && use_quad(st) && use_halo_global(st) && st->npol == 1
+    && hpgx::sub_supported(halo_x(st), halo_y(st))
+    && st->taps_capacity * sizeof(float2) <= size_t(0xffffffffu);
 }
 
 // every CF group has the same support
@@ -360,10 +369,12 @@ int ensure_call_buffers(hpgx_state_t st) {
   if (!cb.hist) {
     const Geometry geo = st->geometry();
     st->nkeys = size_t(geo.ntx) * geo.nty * geo.nr * geo.nc;
-    HPGX_HIP(st, dev_alloc(cb.hist, (st->nkeys + 1) * sizeof(unsigned)));
-    HPGX_HIP(st, dev_alloc(cb.offsets, (st->nkeys + 1) * sizeof(unsigned)));
-    HPGX_HIP(st, dev_alloc(cb.cursor, (st->nkeys + 1) * sizeof(unsigned)));
-    cb.scan_temp_bytes = hpgx::scan_temp_bytes(st->nkeys);
+    // global bin counts for up to 4 keys per (plane, tile) (quadrant keys)
+    const size_t gkeys = 4 * st->nkeys;
+    HPGX_HIP(st, dev_alloc(cb.hist, (gkeys + 1) * sizeof(unsigned)));
+    HPGX_HIP(st, dev_alloc(cb.offsets, (gkeys + 1) * sizeof(unsigned)));
+    HPGX_HIP(st, dev_alloc(cb.cursor, (gkeys + 1) * sizeof(unsigned)));
+    cb.scan_temp_bytes = hpgx::scan_temp_bytes(gkeys);
     HPGX_HIP(st, dev_alloc(cb.scan_temp, cb.scan_temp_bytes));
   }
   if (st->cf.ngroups > 0) {
@@ -804,15 +815,22 @@ int run_grid_with(hpgx_state_t st, CallBuffers& cb, size_t pass_blocks, size_t n
     c.vis = static_cast<char*>(rec_base) + off * rec_bytes;
     c.gvis = cb.gvis + off * np;
     const double2* res = residuals ? residuals + off * np : nullptr;
-    HPGX_HIP(st, hipMemsetAsync(c.hist, 0, (st->nkeys + 1) * sizeof(unsigned), st->stream));
     const bool halo = use_halo_global(st);
+    const bool sub = use_sub_global(st);
+    const size_t keys = sub ? 4 * st->nkeys : st->nkeys;
+    HPGX_HIP(st, hipMemsetAsync(c.hist, 0, (keys + 1) * sizeof(unsigned), st->stream));
     HPGX_HIP(st, hpgx::launch_count(
                    st->stream, np, m, geo, st->cf, st->mu, update_weights, true, c,
-                   st->weights, halo));
-    HPGX_HIP(st, hpgx::launch_scan(st->stream, st->nkeys, c));
+                   st->weights, halo, sub));
+    HPGX_HIP(st, hpgx::launch_scan(st->stream, keys, c));
     HPGX_HIP(st, hpgx::launch_emit(
-                   st->stream, np, m, geo, st->cf, st->mu, gvis_ready, src, res, c, halo));
-    if (halo)
+                   st->stream, np, m, geo, st->cf, st->mu, gvis_ready, src, res, c, halo, sub));
+    if (sub)
+      HPGX_HIP(st, hpgx::launch_tile_sub(
+                     st->stream, np, m, st->nkeys, geo, st->cf,
+                     st->taps_capacity * sizeof(float2), uniform_support(st), st->mu, c, 0,
+                     halo_x(st), halo_y(st), st->grid, true));
+    else if (halo)
       HPGX_HIP(st, hpgx::launch_tile_halo(
                      st->stream, np, mode, st->fp32_tile, m, st->nkeys, geo, st->cf, st->mu,
                      c, 0, halo_x(st), halo_y(st), st->grid, true, use_quad(st)));

@@ -107,16 +107,18 @@ hipError_t launch_scan(hipStream_t s, size_t nkeys, CallBuffers& cb);
 size_t scan_temp_bytes(size_t nkeys);
 // global bins; halo_entries: one Entry4 per record and plane in the bin of
 // the tile holding the support corner (for launch_tile_halo with
-// global_bins), else one Entry per touched tile
+// global_bins; with quad_keys the bin of the corner's quadrant, key 4 x tile
+// key + quadrant, for launch_tile_sub with global_bins), else one Entry per
+// touched tile
 hipError_t launch_count(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool update_weights, bool do_grid, CallBuffers& cb,
-  double* weights, bool halo_entries = false);
+  double* weights, bool halo_entries = false, bool quad_keys = false);
 size_t entry_bytes(int npol);
 hipError_t launch_emit(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool gvis_ready, GvisSource src, const double2* residuals,
-  CallBuffers& cb, bool halo_entries = false);
+  CallBuffers& cb, bool halo_entries = false, bool quad_keys = false);
 hipError_t launch_tile_stream(
   hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid);
@@ -124,6 +126,7 @@ hipError_t launch_tile_stream(
 int bin_blocks_for(size_t n);
 size_t bin_block_records();
 int bin_keys_limit();
+int bin_quad_keys_limit();  // with quad_keys: 4 x (plane, tile) keys
 int tile_bins_limit();
 hipError_t launch_bin(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
@@ -155,11 +158,12 @@ hipError_t launch_tile_halo(
 size_t sub_lds_bytes(int hx, int hy);
 int sub_bins_limit();
 bool sub_supported(int hx, int hy);
-// uni: every CF group has the same support
+// uni: every CF group has the same support; global_bins (one polarization):
+// the entries of launch_emit with quad_keys, cb.offsets their key offsets
 hipError_t launch_tile_sub(
   hipStream_t s, int npol, size_t n, size_t ntile_keys, const Geometry& geo, const DeviceCF& cf,
   size_t taps_bytes, bool uni, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
-  int hx, int hy, void* grid);
+  int hx, int hy, void* grid, bool global_bins = false);
 hipError_t launch_tile_blocal(
   hipStream_t s, int npol, int mode, size_t n, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, size_t block_cap,

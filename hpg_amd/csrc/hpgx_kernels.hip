@@ -294,79 +294,123 @@ struct Entry {
   float2 g[N];
 };
 
+// bin key of tile key k for a support corner (gx, gy): k itself, or with
+// quad_keys 4 k + the quadrant of the corner within its tile
+__device__ __forceinline__ int bin_key(int k, int gx, int gy, int quad_keys) {
+  if (!quad_keys) return k;
+  const int qx = (gx % tile) >= tile / 2 ? 1 : 0;
+  const int qy = (gy % tile) >= tile / 2 ? 1 : 0;
+  return 4 * k + (qx | (qy << 1));
+}
+
 // HALO: one entry per plane, in the bin of the tile holding the support
-// corner (the haloed-tile pipeline); else one per touched tile
+// corner (the haloed-tile pipeline; with quad_keys in the bin of the
+// corner's quadrant, bin_key); else one per touched tile
+// Weights: each wave sums its records' weights per (grid cube, row), and
+// for grids of at most count_planes_lds planes adds the sums into the
+// workgroup's LDS, which adds into the grid weights once at its end (the
+// workgroups stride over the records, so a few thousand global atomics per
+// plane instead of one per wave).
+constexpr int count_planes_lds = 64;
+constexpr unsigned count_blocks_max = 2048;
+
 template <int N, bool HALO>
 __global__ __launch_bounds__(block) void k_count(
   const VisRec<N>* __restrict__ recs, size_t n, Geometry geo,
   const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
   Mueller mu, int update_weights, int do_grid, unsigned* __restrict__ hist,
-  double* __restrict__ weights, unsigned long long* __restrict__ invalid) {
+  double* __restrict__ weights, unsigned long long* __restrict__ invalid, int quad_keys) {
 
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  bool counted = false;
-  unsigned gcube = 0;
-  double tw[4] = {0.0, 0.0, 0.0, 0.0};
-  if (i < n) {
-    const VisRec<N>& r = recs[i];
-    const unsigned grp = r.cf_index[1];
-    const unsigned cfcube = r.cf_index[0];
-    gcube = r.grid_cube;
-    bool valid = grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc);
-    if (valid) valid = cfcube < unsigned(groups[grp].ncube);
-    if (!valid) {
-      atomicAdd(invalid, 1ull);
-    } else {
-      const GroupInfo g = groups[grp];
-      const Coord c = vis_coord(r, geo, g);
-      const bool pos_w = r.uvw[2] > 0;
-      if (c.inside && update_weights) {
-        counted = true;
-        const int fxi = c.fx + geo.F, fyi = c.fy + geo.F;
-        for (int gpol = 0; gpol < geo.nr; ++gpol) {
-          const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
-          double t = 0.0;
-          for (int v = 0; v < N; ++v) {
-            const int m = row[v];
-            double mag = 0.0;
-            if (m >= 0 && m < g.nmueller) {
-              const double2 s = wsum[g.wsum_offset
-                                     + ((long long)(cfcube * g.nmueller + m) * geo.nf
-                                        + fxi) * geo.nf + fyi];
-              mag = hypot(s.x, s.y);
+  __shared__ double wblk[count_planes_lds];
+  const int nplanes = geo.nr * geo.nc;
+  const bool lds_w = update_weights && nplanes <= count_planes_lds;
+  if (lds_w)
+    for (int k = threadIdx.x; k < nplanes; k += blockDim.x) wblk[k] = 0.0;
+  __syncthreads();
+
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  // every thread runs the same number of iterations (the wave reductions
+  // below need all lanes)
+  const size_t iters = (n + stride - 1) / stride;
+  for (size_t it = 0; it < iters; ++it) {
+    const size_t i = i0 + it * stride;
+    bool counted = false;
+    unsigned gcube = 0;
+    double tw[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < n) {
+      const VisRec<N>& r = recs[i];
+      const unsigned grp = r.cf_index[1];
+      const unsigned cfcube = r.cf_index[0];
+      gcube = r.grid_cube;
+      bool valid = grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc);
+      if (valid) valid = cfcube < unsigned(groups[grp].ncube);
+      if (!valid) {
+        atomicAdd(invalid, 1ull);
+      } else {
+        const GroupInfo g = groups[grp];
+        const Coord c = vis_coord(r, geo, g);
+        const bool pos_w = r.uvw[2] > 0;
+        if (c.inside && update_weights) {
+          counted = true;
+          const int fxi = c.fx + geo.F, fyi = c.fy + geo.F;
+          for (int gpol = 0; gpol < geo.nr; ++gpol) {
+            const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
+            double t = 0.0;
+            for (int v = 0; v < N; ++v) {
+              const int m = row[v];
+              double mag = 0.0;
+              if (m >= 0 && m < g.nmueller) {
+                const double2 sw = wsum[g.wsum_offset
+                                        + ((long long)(cfcube * g.nmueller + m) * geo.nf
+                                           + fxi) * geo.nf + fyi];
+                mag = hypot(sw.x, sw.y);
+              }
+              t += mag * double(r.w[v]);
             }
-            t += mag * double(r.w[v]);
+            tw[gpol] = t;
           }
-          tw[gpol] = t;
+        }
+        if (c.inside && do_grid) {
+          const int ntiles = geo.ntx * geo.nty;
+          const int tx0 = tiles_begin(c.gx), tx1 = HALO ? tx0 + 1 : tiles_end(c.gx, g.sx);
+          const int ty0 = tiles_begin(c.gy), ty1 = HALO ? ty0 + 1 : tiles_end(c.gy, g.sy);
+          for (int gpol = 0; gpol < geo.nr; ++gpol) {
+            if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
+            const size_t base = size_t(gcube * geo.nr + gpol) * ntiles;
+            for (int ty = ty0; ty < ty1; ++ty)
+              for (int tx = tx0; tx < tx1; ++tx)
+                atomicAdd(hist + bin_key(int(base) + ty * geo.ntx + tx, c.gx, c.gy, quad_keys),
+                          1u);
+          }
         }
       }
-      if (c.inside && do_grid) {
-        const int ntiles = geo.ntx * geo.nty;
-        const int tx0 = tiles_begin(c.gx), tx1 = HALO ? tx0 + 1 : tiles_end(c.gx, g.sx);
-        const int ty0 = tiles_begin(c.gy), ty1 = HALO ? ty0 + 1 : tiles_end(c.gy, g.sy);
+    }
+    if (update_weights) {
+      unsigned long long active = __ballot(counted);
+      const int lane = threadIdx.x % wave;
+      while (active) {
+        const int leader = __ffsll((long long)active) - 1;
+        const unsigned cube = __shfl(gcube, leader, wave);
+        const bool mine = counted && gcube == cube;
+        active &= ~__ballot(mine);
         for (int gpol = 0; gpol < geo.nr; ++gpol) {
-          if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
-          const size_t base = size_t(gcube * geo.nr + gpol) * ntiles;
-          for (int ty = ty0; ty < ty1; ++ty)
-            for (int tx = tx0; tx < tx1; ++tx)
-              atomicAdd(hist + base + ty * geo.ntx + tx, 1u);
+          const double sw = wave_sum(mine ? tw[gpol] : 0.0);
+          if (lane == leader) {
+            const size_t k = size_t(cube) * geo.nr + gpol;
+            if (lds_w)
+              __hip_atomic_fetch_add(wblk + k, sw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            else
+              atomicAdd(weights + k, sw);
+          }
         }
       }
     }
   }
-  if (update_weights) {
-    unsigned long long active = __ballot(counted);
-    const int lane = threadIdx.x % wave;
-    while (active) {
-      const int leader = __ffsll((long long)active) - 1;
-      const unsigned cube = __shfl(gcube, leader, wave);
-      const bool mine = counted && gcube == cube;
-      active &= ~__ballot(mine);
-      for (int gpol = 0; gpol < geo.nr; ++gpol) {
-        const double s = wave_sum(mine ? tw[gpol] : 0.0);
-        if (lane == leader) atomicAdd(weights + size_t(cube) * geo.nr + gpol, s);
-      }
-    }
+  if (lds_w) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < nplanes; k += blockDim.x)
+      if (wblk[k] != 0.0) atomicAdd(weights + k, wblk[k]);
   }
 }
 
@@ -376,7 +420,7 @@ __global__ __launch_bounds__(block) void k_emit(
   const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
   Mueller mu, int gvis_ready, int adjoint, const float2* __restrict__ gvis_in,
   const double2* __restrict__ residuals, unsigned* __restrict__ cursor,
-  void* __restrict__ entries_out, unsigned* __restrict__ entry_vis) {
+  void* __restrict__ entries_out, unsigned* __restrict__ entry_vis, int quad_keys) {
 
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -424,7 +468,8 @@ __global__ __launch_bounds__(block) void k_emit(
     const size_t base = size_t(gcube * geo.nr + gpol) * ntiles;
     for (int ty = ty0; ty < ty1; ++ty)
       for (int tx = tx0; tx < tx1; ++tx) {
-        const unsigned pos = atomicAdd(cursor + base + ty * geo.ntx + tx, 1u);
+        const unsigned pos =
+          atomicAdd(cursor + bin_key(int(base) + ty * geo.ntx + tx, c.gx, c.gy, quad_keys), 1u);
         if constexpr (HALO) {
           // the haloed-tile entry (layout of Entry4 below): corner relative
           // to the tile, fine offsets, flags
@@ -618,12 +663,13 @@ __global__ __launch_bounds__(block) void k_tile2(
 constexpr int bin_threads = 1024;
 constexpr int bin_recs_per_thread = 4;
 constexpr int bin_records = bin_threads * bin_recs_per_thread;  // per block
-constexpr int bin_keys_max = 8192;
+constexpr int bin_keys_max = 8192;        // (plane, tile) keys of the block-local bins
+constexpr int bin_quad_keys_max = 16384;  // quadrant keys (4 per tile key): LDS histogram
 constexpr int bin_planes_max = 1024;
 constexpr int tile_bins_max = 2048;  // bin blocks per pass (LDS of k_tile3)
 constexpr int bin_stage_bytes = 32768;  // LDS staging of k_bin's haloed entries
 
-// block-wide exclusive scan of a[0, n) in LDS (n <= bin_keys_max), returns
+// block-wide exclusive scan of a[0, n) in LDS (n <= bin_quad_keys_max), returns
 // the total; `part` holds blockDim.x / wave + 1 values
 __device__ unsigned block_exclusive_scan(unsigned* a, int n, unsigned* part) {
   const int nt = blockDim.x;
@@ -680,15 +726,6 @@ constexpr unsigned e4_gradient = 1u << 29;
 // k_emit<N, true> writes this layout as 2 + 2N 32-bit words
 static_assert(sizeof(Entry4<1>) == 16 && sizeof(Entry4<4>) == 40, "Entry4 layout");
 
-// bin key of tile key k for a support corner (gx, gy): k itself, or with
-// quad_keys 4 k + the quadrant of the corner within its tile
-__device__ __forceinline__ int bin_key(int k, int gx, int gy, int quad_keys) {
-  if (!quad_keys) return k;
-  const int qx = (gx % tile) >= tile / 2 ? 1 : 0;
-  const int qy = (gy % tile) >= tile / 2 ? 1 : 0;
-  return 4 * k + (qx | (qy << 1));
-}
-
 // HALO = false: one entry per (plane, tile) the support touches (Entry<N>,
 // for k_tile3); HALO = true: one entry per plane, keyed by the tile of the
 // support corner (Entry4<N>, for k_tile4)
@@ -702,7 +739,7 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
   unsigned* __restrict__ entry_vis, size_t block_cap, double* __restrict__ weights,
   unsigned long long* __restrict__ invalid, VisRec<N>* __restrict__ copy_recs, int quad_keys) {
 
-  __shared__ unsigned cnt[bin_keys_max];
+  __shared__ unsigned cnt[bin_quad_keys_max];
   __shared__ double wl[bin_planes_max];
   __shared__ unsigned part[bin_threads / wave + 1];
 
@@ -2368,8 +2405,11 @@ __device__ __forceinline__ void sub_gradient_entry(
 // PX: region pixels per lane; each wave's region is padded to PX * 64
 // pixels (>= (16 + hx) * (16 + hy)), so a lane folds pixels lane + 64 k with
 // no bounds test.  UNI: every CF group has the same support, whose lane mask
-// then holds for the whole call.  PK: see sub_pk_flip.
-template <int N, int PX, bool UNI, bool PK>
+// then holds for the whole call.  PK: see sub_pk_flip.  GB (global bins,
+// grids of more keys than one workgroup's LDS histogram holds): off_table
+// is the exclusive scan of the quadrant keys' entry counts (k_count and
+// k_emit with quad_keys), each key's entries one contiguous range.
+template <int N, int PX, bool UNI, bool PK, bool GB>
 __global__ __launch_bounds__(sub_block) __attribute__((amdgpu_waves_per_eu(N == 1 ? 8 : 4)))
 void k_sub(
   Geometry geo, const GroupInfo* __restrict__ groups, const float2* __restrict__ taps,
@@ -2383,6 +2423,7 @@ void k_sub(
   extern __shared__ float2 sregions[];
   __shared__ unsigned seg_pref[4][sub_bins_max + 1];
   __shared__ unsigned short seg_beg[4][sub_bins_max];
+  __shared__ unsigned gb_beg[4];  // GB: first entry of each quadrant key
   constexpr int QP = PX * wave;
 
   const int QX = quad_edge + hx;
@@ -2397,6 +2438,17 @@ void k_sub(
   const int q = w & 3;
   const int half = w >> 2;
 
+  if constexpr (GB) {
+    // one range per quadrant key: seg_pref[q][nbins = 1] its length
+    if (tid < 4) {
+      const unsigned s0 = off_table[4 * size_t(tkey) + tid];
+      const unsigned s1 = off_table[4 * size_t(tkey) + tid + 1];
+      gb_beg[tid] = s0;
+      seg_pref[tid][0] = 0u;
+      seg_pref[tid][1] = s1 - s0;
+    }
+    __syncthreads();
+  } else {
   // segment tables of the four quadrant keys, then their exclusive scans
   for (int i = int(tid); i < 4 * nbins; i += sub_block) {
     const int bb = i >> 2, qq = i & 3;
@@ -2432,6 +2484,7 @@ void k_sub(
     if (lane == wave - 1) a[nbins] = incl;
   }
   __syncthreads();
+  }
   if (seg_pref[0][nbins] + seg_pref[1][nbins] + seg_pref[2][nbins] + seg_pref[3][nbins] == 0u) {
     if (tid == 0) halo_set[tkey] = 0u;
     return;
@@ -2500,6 +2553,7 @@ void k_sub(
   // costs outweigh its balance)
   const unsigned total = seg_pref[q][nbins];
   auto locate = [&](unsigned i) {
+    if constexpr (GB) return gb_beg[q] + i;
     int lo = 0, hi = nbins;  // seg_pref[q][lo] <= i < seg_pref[q][hi]
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
@@ -3426,13 +3480,15 @@ hipError_t launch_scan(hipStream_t s, size_t nkeys, CallBuffers& cb) {
 hipError_t launch_count(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool update_weights, bool do_grid, CallBuffers& cb,
-  double* weights, bool halo_entries) {
+  double* weights, bool halo_entries, bool quad_keys) {
   if (n == 0) return hipSuccess;
 #define HPGX_COUNT(NP, H)                                                      \
   hipLaunchKernelGGL(                                                          \
-    (k_count<NP, H>), dim3(blocks_for(n)), dim3(block), 0, s,                  \
+    (k_count<NP, H>), dim3(std::min(unsigned(blocks_for(n)), count_blocks_max)), \
+    dim3(block), 0, s,                                                         \
     static_cast<const VisRec<NP>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,    \
-    int(update_weights), int(do_grid), cb.hist, weights, cb.invalid_count)
+    int(update_weights), int(do_grid), cb.hist, weights, cb.invalid_count,     \
+    int(halo_entries && quad_keys))
   if (halo_entries) {
     switch (npol) {
     case 1: HPGX_COUNT(1, true); break;
@@ -3459,7 +3515,7 @@ size_t entry_bytes(int npol) { return sizeof(EntryHead) + sizeof(float2) * size_
 hipError_t launch_emit(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool gvis_ready, GvisSource src, const double2* residuals,
-  CallBuffers& cb, bool halo_entries) {
+  CallBuffers& cb, bool halo_entries, bool quad_keys) {
   if (n == 0) return hipSuccess;
   const int adj = src == GvisSource::Adjoint;
 #define HPGX_EMIT(NP, H)                                                       \
@@ -3467,7 +3523,7 @@ hipError_t launch_emit(
     (k_emit<NP, H>), dim3(blocks_for(n)), dim3(block), 0, s,                   \
     static_cast<const VisRec<NP>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,    \
     int(gvis_ready), adj, cb.gvis, residuals, cb.cursor, cb.stream_entries,    \
-    cb.entry_vis)
+    cb.entry_vis, int(halo_entries && quad_keys))
   if (halo_entries) {
     switch (npol) {
     case 1: HPGX_EMIT(1, true); break;
@@ -3523,6 +3579,7 @@ hipError_t launch_tile_stream(
 int bin_blocks_for(size_t n) { return int((n + bin_records - 1) / bin_records); }
 size_t bin_block_records() { return bin_records; }
 int bin_keys_limit() { return bin_keys_max; }
+int bin_quad_keys_limit() { return bin_quad_keys_max; }
 int tile_bins_limit() { return tile_bins_max; }
 
 hipError_t launch_bin(
@@ -3623,28 +3680,35 @@ bool sub_supported(int hx, int hy) { return hx >= 0 && hy >= 0 && hx < 8 && hy <
 hipError_t launch_tile_sub(
   hipStream_t s, int npol, size_t n, size_t ntile_keys, const Geometry& geo, const DeviceCF& cf,
   size_t taps_bytes, bool uni, const Mueller& mu, const CallBuffers& cb, size_t block_cap,
-  int hx, int hy, void* grid) {
+  int hx, int hy, void* grid, bool global_bins) {
   if (ntile_keys == 0 || n == 0) return hipSuccess;
-  const int nb = bin_blocks_for(n);
+  const int nb = global_bins ? 1 : bin_blocks_for(n);
   // segment starts are 16-bit offsets within a bin block; CF taps are
-  // addressed by 32-bit byte offsets
-  if (nb > sub_bins_max || !sub_supported(hx, hy) || block_cap > 65536
-      || taps_bytes > 0xffffffffull)
+  // addressed by 32-bit byte offsets; global bins for one polarization
+  if (!sub_supported(hx, hy) || taps_bytes > 0xffffffffull) return hipErrorInvalidValue;
+  if (global_bins ? npol != 1 : (nb > sub_bins_max || block_cap > 65536))
     return hipErrorInvalidValue;
+  const unsigned* offs = global_bins ? cb.offsets : cb.off_table;
   const int Q = (quad_edge + hx) * (quad_edge + hy);
   const int px = (Q + wave - 1) / wave;  // 4 .. 9
   const int pxc = px <= 4 ? 4 : px <= 6 ? 6 : px <= 8 ? 8 : 9;
   const size_t dyn = size_t(sub_waves) * pxc * wave * sizeof(float2);
-#define HPGX_SUB(NP, PX, PK)                                                   \
+#define HPGX_SUB_G(NP, PX, PK, GB)                                             \
   hipLaunchKernelGGL(                                                          \
-    uni ? (k_sub<NP, PX, true, PK>) : (k_sub<NP, PX, false, PK>),              \
+    uni ? (k_sub<NP, PX, true, PK, GB>) : (k_sub<NP, PX, false, PK, GB>),      \
     dim3(unsigned(ntile_keys)), dim3(sub_block), dyn, s,                       \
-    geo, cf.groups, cf.taps, unsigned(taps_bytes), mu, cb.off_table, nb,       \
+    geo, cf.groups, cf.taps, unsigned(taps_bytes), mu, offs, nb,               \
     block_cap,                                                                 \
     static_cast<const Entry4<NP>*>(cb.stream_entries), cb.entry_vis,           \
     static_cast<const VisRec<NP>*>(cb.vis), hx, hy, static_cast<float2*>(grid),\
     cb.halo, cb.halo_set)
-#define HPGX_SUB_PX(NP, PK)                                                    \
+#define HPGX_SUB_N(NP, PX, PK) HPGX_SUB_G(NP, PX, PK, false)
+#define HPGX_SUB_1(NP, PX, PK)                                                 \
+  do {                                                                         \
+    if (global_bins) HPGX_SUB_G(1, PX, PK, true);                              \
+    else HPGX_SUB_G(1, PX, PK, false);                                         \
+  } while (0)
+#define HPGX_SUB_PX(HPGX_SUB, NP, PK)                                          \
   switch (pxc) {                                                               \
   case 4: HPGX_SUB(NP, 4, PK); break;                                          \
   case 6: HPGX_SUB(NP, 6, PK); break;                                          \
@@ -3657,15 +3721,17 @@ hipError_t launch_tile_sub(
   const bool pk = npol == 1 && taps_bytes / sizeof(float2) < sub_pk_taps;
   switch (npol) {
   case 1:
-    if (pk) HPGX_SUB_PX(1, true) else HPGX_SUB_PX(1, false)
+    if (pk) HPGX_SUB_PX(HPGX_SUB_1, 1, true) else HPGX_SUB_PX(HPGX_SUB_1, 1, false)
     break;
-  case 2: HPGX_SUB_PX(2, false); break;
-  case 3: HPGX_SUB_PX(3, false); break;
-  case 4: HPGX_SUB_PX(4, false); break;
+  case 2: HPGX_SUB_PX(HPGX_SUB_N, 2, false); break;
+  case 3: HPGX_SUB_PX(HPGX_SUB_N, 3, false); break;
+  case 4: HPGX_SUB_PX(HPGX_SUB_N, 4, false); break;
   default: return hipErrorInvalidValue;
   }
 #undef HPGX_SUB_PX
-#undef HPGX_SUB
+#undef HPGX_SUB_1
+#undef HPGX_SUB_N
+#undef HPGX_SUB_G
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return launch_halo(s, 1, ntile_keys, geo, cb, hx, hy, grid);

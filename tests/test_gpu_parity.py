@@ -213,6 +213,39 @@ def test_quadrant_kernels_vs_oracle(hpg, oracle_mod, monkeypatch, sub, case):
     np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
 
 
+GLOBAL_SUB_RADII = [[3], [1, (3, 2), 2], [1]]
+
+
+@pytest.mark.parametrize("case", range(len(GLOBAL_SUB_RADII)))
+def test_quadrant_kernel_global_bins_vs_oracle(hpg, oracle_mod, case):
+    """The quadrant-key kernel on globally counted quadrant bins: a
+    one-polarization FP32-compensated grid of 320 x 288 x 1 x 100 (9000
+    (plane, tile) keys, more than one workgroup's histogram holds), one
+    support or several, phase gradients on a third of the records, records
+    off the grid: e_max, e_rms <= 1e-6 per plane, weights to 1e-12."""
+    from hpg_amd import workloads
+    gsz = (320, 288, 1, 100)
+    cf = workloads.random_cf(7, GLOBAL_SUB_RADII[case], nmueller=1, ncube=2, seed=190 + case)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (0.9, 1.1), [[0]], cf, 1)
+    rec = workloads.random_visibilities(20000, 1, gsz, (0.9, 1.1), cf, seed=195 + case,
+                                        phase=True, margin=-3, gradient=True)
+    rec["cf_phase_gradient"][1::3] = 0.0
+    rec["cf_phase_gradient"][2::3] = 0.0
+    gs.grid_visibilities(hpg.Device.Serial, rec, True)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    got, exp = gs.grid_values(), o.grid_values()
+    planes = 0
+    for c in range(gsz[3]):
+        if np.abs(exp[c, 0]).max() == 0:
+            assert np.abs(got[c, 0]).max() == 0
+            continue
+        planes += 1
+        e_max, e_rms = rel_errors(got[c, 0], exp[c, 0])
+        assert e_max <= 1e-6 and e_rms <= 1e-6, (c, e_max, e_rms)
+    assert planes >= 90
+    np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
+
+
 @pytest.mark.parametrize("gridder", [2, 3, 4, 5])
 @pytest.mark.parametrize("case", [0, 2, 3, 4])
 @pytest.mark.parametrize("mode", [0, 1])
