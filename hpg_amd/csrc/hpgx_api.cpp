@@ -423,6 +423,14 @@ int ensure_call_buffers(hpgx_state_t st) {
         cb.halo_set_capacity = st->nkeys;
       }
     }
+    if (use_sub_global(st) && need > cb.tally_capacity) {
+      dev_free(cb.tally_entries);
+      dev_free(cb.tally_keys);
+      cb.tally_capacity = 0;
+      HPGX_HIP(st, dev_alloc(cb.tally_entries, need * hpgx::halo_entry_bytes(int(st->npol))));
+      HPGX_HIP(st, dev_alloc(cb.tally_keys, need * 2 * sizeof(unsigned)));
+      cb.tally_capacity = need;
+    }
     if (need > cb.entries_capacity) {
       dev_free(cb.stream_entries);
       dev_free(cb.entry_vis);
@@ -466,6 +474,8 @@ void free_state(hpgx_state_t st) {
     if (e) (void)hipEventDestroy(e);
   dev_free(cb.vis);
   dev_free(cb.gvis);
+  dev_free(cb.tally_entries);
+  dev_free(cb.tally_keys);
   dev_free(cb.hist);
   dev_free(cb.offsets);
   dev_free(cb.cursor);
@@ -807,7 +817,12 @@ int run_grid_with(hpgx_state_t st, CallBuffers& cb, size_t pass_blocks, size_t n
     }
     return HPGX_SUCCESS;
   }
-  const size_t chunk = std::max<size_t>(1, cb.entries_capacity / epr);
+  // records per pass: their entries fit the entry buffer (and, through
+  // launch_tally, its record-order slots)
+  const size_t slots =
+    use_sub_global(st) ? std::min(cb.entries_capacity, cb.tally_capacity) : cb.entries_capacity;
+  if (slots < epr) return fail(st, HPGX_ERR_OTHER, "bin entry buffers not allocated");
+  const size_t chunk = slots / epr;
   const size_t rec_bytes = visdata_size(st->npol);
   for (size_t off = 0; off < n; off += chunk) {
     const size_t m = std::min(chunk, n - off);
@@ -819,12 +834,20 @@ int run_grid_with(hpgx_state_t st, CallBuffers& cb, size_t pass_blocks, size_t n
     const bool sub = use_sub_global(st);
     const size_t keys = sub ? 4 * st->nkeys : st->nkeys;
     HPGX_HIP(st, hipMemsetAsync(c.hist, 0, (keys + 1) * sizeof(unsigned), st->stream));
-    HPGX_HIP(st, hpgx::launch_count(
-                   st->stream, np, m, geo, st->cf, st->mu, update_weights, true, c,
-                   st->weights, halo, sub));
-    HPGX_HIP(st, hpgx::launch_scan(st->stream, keys, c));
-    HPGX_HIP(st, hpgx::launch_emit(
-                   st->stream, np, m, geo, st->cf, st->mu, gvis_ready, src, res, c, halo, sub));
+    if (sub) {
+      HPGX_HIP(st, hpgx::launch_tally(
+                     st->stream, np, m, geo, st->cf, st->mu, update_weights, gvis_ready, src,
+                     res, c, st->weights));
+      HPGX_HIP(st, hpgx::launch_scan(st->stream, keys, c));
+      HPGX_HIP(st, hpgx::launch_place(st->stream, np, m, geo, c));
+    } else {
+      HPGX_HIP(st, hpgx::launch_count(
+                     st->stream, np, m, geo, st->cf, st->mu, update_weights, true, c,
+                     st->weights, halo));
+      HPGX_HIP(st, hpgx::launch_scan(st->stream, keys, c));
+      HPGX_HIP(st, hpgx::launch_emit(
+                     st->stream, np, m, geo, st->cf, st->mu, gvis_ready, src, res, c, halo));
+    }
     if (sub)
       HPGX_HIP(st, hpgx::launch_tile_sub(
                      st->stream, np, m, st->nkeys, geo, st->cf,

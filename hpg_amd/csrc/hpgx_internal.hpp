@@ -81,6 +81,9 @@ struct CallBuffers {
   unsigned long long* invalid_count;
   double2* degrid_acc;   // binned degridding: per record x polarization sums
   size_t degrid_acc_capacity;
+  void* tally_entries;   // launch_tally: entries in record order (Entry4<N>)
+  void* tally_keys;      // launch_tally: (key, place within the key) per slot
+  size_t tally_capacity; // slots of both
 };
 
 struct DeviceCF {
@@ -119,6 +122,16 @@ hipError_t launch_emit(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, bool gvis_ready, GvisSource src, const double2* residuals,
   CallBuffers& cb, bool halo_entries = false, bool quad_keys = false);
+// global quadrant bins (one polarization, haloed entries) in two passes:
+// launch_tally (counts in cb.hist, entries and their (key, place) in
+// record order), launch_scan, then launch_place (entries into their keys'
+// ranges); n x nr slots of cb.tally_entries / cb.tally_keys
+hipError_t launch_tally(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, bool update_weights, bool gvis_ready, GvisSource src,
+  const double2* residuals, CallBuffers& cb, double* weights);
+hipError_t launch_place(hipStream_t s, int npol, size_t n, const Geometry& geo,
+                        CallBuffers& cb);
 hipError_t launch_tile_stream(
   hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid);

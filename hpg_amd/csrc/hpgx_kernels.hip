@@ -294,6 +294,22 @@ struct Entry {
   float2 g[N];
 };
 
+// entries of the haloed-tile pipeline (k_tile4): one per (record, plane),
+// position relative to the tile that holds the support corner
+struct Entry4Head {
+  unsigned pos;  // lx | ly << 6 | fxi << 12 | fyi << 20 | flags << 28
+  unsigned gpk;  // grp | cfcube << 16
+};
+template <int N>
+struct Entry4 {
+  Entry4Head h;
+  float2 g[N];
+};
+constexpr unsigned e4_pos_w = 1u << 28;
+constexpr unsigned e4_gradient = 1u << 29;
+// k_emit<N, true> writes this layout as 2 + 2N 32-bit words
+static_assert(sizeof(Entry4<1>) == 16 && sizeof(Entry4<4>) == 40, "Entry4 layout");
+
 // bin key of tile key k for a support corner (gx, gy): k itself, or with
 // quad_keys 4 k + the quadrant of the corner within its tile
 __device__ __forceinline__ int bin_key(int k, int gx, int gy, int quad_keys) {
@@ -488,6 +504,154 @@ __global__ __launch_bounds__(block) void k_emit(
         if (grad) entry_vis[pos] = unsigned(i);
       }
   }
+}
+
+// Global quadrant bins in two passes over the records instead of the three
+// of k_count + k_emit: k_tally makes each record's entries (one per grid row
+// with a Mueller index, keyed by the quadrant of the support corner) once,
+// takes each entry's place within its key from the returning count, and
+// writes the entry and its (key, place) in record order (slot
+// record * nr + row); after the scan of the counts, k_place moves each
+// entry into its key's contiguous range.  Weights as k_count.
+template <int N>
+__global__ __launch_bounds__(block) void k_tally(
+  const VisRec<N>* __restrict__ recs, size_t n, Geometry geo,
+  const GroupInfo* __restrict__ groups, const double2* __restrict__ wsum,
+  Mueller mu, int update_weights, int gvis_ready, int adjoint,
+  const float2* __restrict__ gvis_in, const double2* __restrict__ residuals,
+  unsigned* __restrict__ hist, double* __restrict__ weights,
+  unsigned long long* __restrict__ invalid, Entry4<N>* __restrict__ tally,
+  uint2* __restrict__ tally_keys) {
+
+  __shared__ double wblk[count_planes_lds];
+  const int nplanes = geo.nr * geo.nc;
+  const bool lds_w = update_weights && nplanes <= count_planes_lds;
+  if (lds_w)
+    for (int k = threadIdx.x; k < nplanes; k += blockDim.x) wblk[k] = 0.0;
+  __syncthreads();
+
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t iters = (n + stride - 1) / stride;
+  const int ntiles = geo.ntx * geo.nty;
+  for (size_t it = 0; it < iters; ++it) {
+    const size_t i = i0 + it * stride;
+    bool counted = false;
+    unsigned gcube = 0;
+    double tw[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < n) {
+      // slots of rows without an entry say so
+      for (int gpol = 0; gpol < geo.nr; ++gpol)
+        tally_keys[i * geo.nr + gpol] = make_uint2(~0u, 0u);
+      const VisRec<N>& r = recs[i];
+      const unsigned grp = r.cf_index[1];
+      const unsigned cfcube = r.cf_index[0];
+      gcube = r.grid_cube;
+      bool valid = grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc);
+      if (valid) valid = cfcube < unsigned(groups[grp].ncube);
+      if (!valid) {
+        atomicAdd(invalid, 1ull);
+      } else {
+        const GroupInfo g = groups[grp];
+        const Coord c = vis_coord(r, geo, g);
+        const bool pos_w = r.uvw[2] > 0;
+        const int fxi = c.fx + geo.F, fyi = c.fy + geo.F;
+        if (c.inside && update_weights) {
+          counted = true;
+          for (int gpol = 0; gpol < geo.nr; ++gpol) {
+            const int* row = pos_w ? mu.M[gpol] : mu.Mc[gpol];
+            double t = 0.0;
+            for (int v = 0; v < N; ++v) {
+              const int m = row[v];
+              double mag = 0.0;
+              if (m >= 0 && m < g.nmueller) {
+                const double2 sw = wsum[g.wsum_offset
+                                        + ((long long)(cfcube * g.nmueller + m) * geo.nf
+                                           + fxi) * geo.nf + fyi];
+                mag = hypot(sw.x, sw.y);
+              }
+              t += mag * double(r.w[v]);
+            }
+            tw[gpol] = t;
+          }
+        }
+        if (c.inside) {
+          const bool grad = r.grad[0] != 0.0 || r.grad[1] != 0.0;
+          Entry4<N> e;
+          const int tx = tiles_begin(c.gx), ty = tiles_begin(c.gy);
+          e.h.pos = unsigned(c.gx - tx * tile) | (unsigned(c.gy - ty * tile) << 6)
+            | (unsigned(fxi) << 12) | (unsigned(fyi) << 20) | (pos_w ? e4_pos_w : 0u)
+            | (grad ? e4_gradient : 0u);
+          e.h.gpk = grp | (cfcube << 16);
+          if (gvis_ready) {
+            for (int v = 0; v < N; ++v) e.g[v] = gvis_in[i * N + v];
+          } else {
+            const double2 phasor = cphase(r.phase);
+            if (!adjoint) {
+              for (int v = 0; v < N; ++v) e.g[v] = prephase(r.vis[v], phasor, r.w[v]);
+            } else {
+              double2 W[N];
+              degrid_norms<N>(geo, g, wsum, mu, pos_w, cfcube, fxi, fyi, W);
+              for (int v = 0; v < N; ++v) {
+                double2 d = W[v];
+                if (d.x == 0.0 && d.y == 0.0) d = make_double2(1.0, 0.0);
+                d.y = -d.y;
+                e.g[v] = narrow(kdiv(cmul(residuals[i * N + v], phasor), d));
+              }
+            }
+          }
+          for (int gpol = 0; gpol < geo.nr; ++gpol) {
+            if (!row_has_index(pos_w ? mu.M[gpol] : mu.Mc[gpol], N)) continue;
+            const int key =
+              bin_key((int(gcube) * geo.nr + gpol) * ntiles + ty * geo.ntx + tx, c.gx, c.gy, 1);
+            const unsigned place = atomicAdd(hist + key, 1u);
+            tally[i * geo.nr + gpol] = e;
+            tally_keys[i * geo.nr + gpol] = make_uint2(unsigned(key), place);
+          }
+        }
+      }
+    }
+    if (update_weights) {
+      unsigned long long active = __ballot(counted);
+      const int lane = threadIdx.x % wave;
+      while (active) {
+        const int leader = __ffsll((long long)active) - 1;
+        const unsigned cube = __shfl(gcube, leader, wave);
+        const bool mine = counted && gcube == cube;
+        active &= ~__ballot(mine);
+        for (int gpol = 0; gpol < geo.nr; ++gpol) {
+          const double sw = wave_sum(mine ? tw[gpol] : 0.0);
+          if (lane == leader) {
+            const size_t k = size_t(cube) * geo.nr + gpol;
+            if (lds_w)
+              __hip_atomic_fetch_add(wblk + k, sw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            else
+              atomicAdd(weights + k, sw);
+          }
+        }
+      }
+    }
+  }
+  if (lds_w) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < nplanes; k += blockDim.x)
+      if (wblk[k] != 0.0) atomicAdd(weights + k, wblk[k]);
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(block) void k_place(
+  size_t slots, int nr, const unsigned* __restrict__ offsets,
+  const Entry4<N>* __restrict__ tally, const uint2* __restrict__ tally_keys,
+  Entry4<N>* __restrict__ entries, unsigned* __restrict__ entry_vis) {
+  const size_t sl = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (sl >= slots) return;
+  const uint2 kp = tally_keys[sl];
+  if (kp.x == ~0u) return;
+  const unsigned pos = offsets[kp.x] + kp.y;
+  const Entry4<N> e = tally[sl];
+  entries[pos] = e;
+  if (e.h.pos & e4_gradient) entry_vis[pos] = unsigned(sl / size_t(nr));
 }
 
 __device__ __forceinline__ int bcast(int v, int k) {
@@ -710,21 +874,6 @@ __device__ unsigned block_exclusive_scan(unsigned* a, int n, unsigned* part) {
   return total;
 }
 
-// entries of the haloed-tile pipeline (k_tile4): one per (record, plane),
-// position relative to the tile that holds the support corner
-struct Entry4Head {
-  unsigned pos;  // lx | ly << 6 | fxi << 12 | fyi << 20 | flags << 28
-  unsigned gpk;  // grp | cfcube << 16
-};
-template <int N>
-struct Entry4 {
-  Entry4Head h;
-  float2 g[N];
-};
-constexpr unsigned e4_pos_w = 1u << 28;
-constexpr unsigned e4_gradient = 1u << 29;
-// k_emit<N, true> writes this layout as 2 + 2N 32-bit words
-static_assert(sizeof(Entry4<1>) == 16 && sizeof(Entry4<4>) == 40, "Entry4 layout");
 
 // HALO = false: one entry per (plane, tile) the support touches (Entry<N>,
 // for k_tile3); HALO = true: one entry per plane, keyed by the tile of the
@@ -3511,6 +3660,33 @@ hipError_t launch_count(
 }
 
 size_t entry_bytes(int npol) { return sizeof(EntryHead) + sizeof(float2) * size_t(npol); }
+
+hipError_t launch_tally(
+  hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
+  const Mueller& mu, bool update_weights, bool gvis_ready, GvisSource src,
+  const double2* residuals, CallBuffers& cb, double* weights) {
+  if (n == 0) return hipSuccess;
+  if (npol != 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(
+    k_tally<1>, dim3(std::min(blocks_for(n), count_blocks_max)), dim3(block), 0, s,
+    static_cast<const VisRec<1>*>(cb.vis), n, geo, cf.groups, cf.wsum, mu,
+    int(update_weights), int(gvis_ready), int(src == GvisSource::Adjoint), cb.gvis, residuals,
+    cb.hist, weights, cb.invalid_count, static_cast<Entry4<1>*>(cb.tally_entries),
+    static_cast<uint2*>(cb.tally_keys));
+  return hipGetLastError();
+}
+
+hipError_t launch_place(hipStream_t s, int npol, size_t n, const Geometry& geo,
+                        CallBuffers& cb) {
+  const size_t slots = n * size_t(geo.nr);
+  if (slots == 0) return hipSuccess;
+  if (npol != 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(
+    k_place<1>, dim3(blocks_for(slots)), dim3(block), 0, s, slots, geo.nr, cb.offsets,
+    static_cast<const Entry4<1>*>(cb.tally_entries), static_cast<const uint2*>(cb.tally_keys),
+    static_cast<Entry4<1>*>(cb.stream_entries), cb.entry_vis);
+  return hipGetLastError();
+}
 
 hipError_t launch_emit(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
