@@ -945,6 +945,15 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
   static_assert(sizeof(VisRec<N>) % sizeof(unsigned long long) == 0, "8-byte record words");
   static_assert(bin_recs_per_thread % 2 == 0, "records loaded in pairs");
   VisRec<N> rc[hoist ? 2 : 1];
+  // HALO: the place of each of this thread's entries (one per record and
+  // grid row with an index; ~0u for none) within its key, from the count's
+  // returned value, then (after the scan) its block-local position; the
+  // entries are then written through LDS in position order
+  unsigned hpos[bin_recs_per_thread][4];
+#pragma unroll
+  for (int j = 0; j < bin_recs_per_thread; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) hpos[j][r] = ~0u;
 
 #pragma unroll
   for (int j = 0; j < bin_recs_per_thread; ++j) {
@@ -1066,10 +1075,18 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
       if (!((rows >> gpol) & 1u)) continue;
       const int base = (int(gcube) * geo.nr + gpol) * ntiles;
       for (int ty = ty0; ty < ty1; ++ty)
-        for (int tx = tx0; tx < tx1; ++tx)
-          __hip_atomic_fetch_add(
+        for (int tx = tx0; tx < tx1; ++tx) {
+          const unsigned place = __hip_atomic_fetch_add(
             cnt + bin_key(base + ty * geo.ntx + tx, c.gx, c.gy, quad_keys), 1u,
             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if constexpr (HALO) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              if (r == gpol) hpos[j][r] = place;
+          } else {
+            (void)place;
+          }
+        }
     }
   }
   // the threads' weight sums: one sum per (wave, grid cube) present, the
@@ -1107,14 +1124,6 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
   __syncthreads();
 
   unsigned* outv = entry_vis + b * block_cap;
-  // HALO: the block-local position of each of this thread's entries (one
-  // per record and grid row with an index; ~0u for none); the entries are
-  // then written through LDS in position order
-  unsigned hpos[bin_recs_per_thread][4];
-#pragma unroll
-  for (int j = 0; j < bin_recs_per_thread; ++j)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) hpos[j][r] = ~0u;
 #pragma unroll
   for (int j = 0; j < bin_recs_per_thread; ++j) {
     if (!keep[j]) continue;
@@ -1131,14 +1140,21 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
       const int base = (int(gc[j]) * geo.nr + gpol) * ntiles;
       for (int ty = ty0; ty < ty1; ++ty)
         for (int tx = tx0; tx < tx1; ++tx) {
-          const unsigned pos = __hip_atomic_fetch_add(
-            cnt + bin_key(base + ty * geo.ntx + tx, e[j].h.gx, e[j].h.gy, quad_keys), 1u,
-            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const int key = bin_key(base + ty * geo.ntx + tx, e[j].h.gx, e[j].h.gy, quad_keys);
+          unsigned pos;
           if constexpr (HALO) {
+            // the key's offset (cnt holds the scan now) + the entry's place
+            unsigned place = 0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              if (r == gpol) place = hpos[j][r];
+            pos = cnt[key] + place;
 #pragma unroll
             for (int r = 0; r < 4; ++r)
               if (r == gpol) hpos[j][r] = pos;
           } else {
+            pos = __hip_atomic_fetch_add(cnt + key, 1u, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
             static_cast<Entry<N>*>(entries_out)[b * block_cap + pos] = e[j];
           }
           if (grad || degrid_pass) outv[pos] = unsigned(b * bin_records + i);
