@@ -661,7 +661,15 @@ __device__ __forceinline__ float bcast(float v, int k) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-template <int N, int MODE>
+// k_tile2 with UNR: tap iterations whose loads are issued together (two for
+// four polarizations, whose loads would otherwise take the registers of
+// several waves).  The unrolled kernel holds fewer waves, which pays for
+// large supports only (unroll_min_taps and more taps).
+template <int N>
+constexpr int tap_unroll() { return N <= 2 ? 4 : 2; }
+constexpr int unroll_min_taps = 2048;
+
+template <int N, int MODE, bool UNR>
 __global__ __launch_bounds__(block) void k_tile2(
   Geometry geo, const GroupInfo* __restrict__ groups,
   const float2* __restrict__ taps, Mueller mu,
@@ -749,36 +757,116 @@ __global__ __launch_bounds__(block) void k_tile2(
         dphiy = gyr * geo.os;
       }
 
-      for (int q = lane; q < ntap; q += wave) {
-        const int yl = int(div_magic(unsigned(q), unsigned(wdt), mg));
-        const int X = xa + (q - yl * wdt);
-        const int Y = ya + yl;
-        float2 scr = make_float2(1.0f, 0.0f);
+      if constexpr (!UNR) {
+        for (int q = lane; q < ntap; q += wave) {
+          const int yl = int(div_magic(unsigned(q), unsigned(wdt), mg));
+          const int X = xa + (q - yl * wdt);
+          const int Y = ya + yl;
+          float2 scr = make_float2(1.0f, 0.0f);
+          if (grad) {
+            const double phX = phi0x + X * dphix;
+            const double phY = phi0y + Y * dphiy;
+            scr = narrow(cphase(phX + phY));
+          }
+          const int li = (gy + Y - y0) * tile + (gx + X - x0);
+          double sr = 0.0, si = 0.0;
+          for (int v = 0; v < N; ++v) {
+            if (!use[v]) continue;
+            float2 c = slice[v][Y * g.sx + X];
+            c.y *= imf;
+            if (grad) c = cmul(c, scr);
+            const float2 pr = cmul(c, gv[v]);
+            sr += double(pr.x);
+            si += double(pr.y);
+          }
+          if constexpr (MODE == 0) {
+            __hip_atomic_fetch_add(acc64 + li, sr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(acc64 + TT + li, si, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          } else {
+            // split the FP64 tap sum into float hi + lo (exact for N == 1)
+            const float hr = float(sr), hi = float(si);
+            const float lr = float(sr - double(hr)), lim = float(si - double(hi));
+            pair_add<(N > 1)>(pairs + li, pairs[li], hr, lr);
+            pair_add<(N > 1)>(pairs + TT + li, pairs[TT + li], hi, lim);
+          }
+        }
+        continue;
+      }
+      // the footprint tile_unroll x 64 taps at a time: the CF loads of all
+      // of them issued before the first is used (one step for footprints of
+      // at most 64 taps, and for records with a phase gradient: their FP64
+      // screen would not fit the registers of the unrolled loop)
+      constexpr int tile_unroll = tap_unroll<N>();
+      const int unroll = grad || ntap <= wave ? 1 : tile_unroll;
+      for (int q0 = lane; q0 < ntap; q0 += unroll * wave) {
+        float2 cv[tile_unroll][N];
+        int tx[tile_unroll], ty[tile_unroll];
+#pragma unroll
+        for (int u = 0; u < tile_unroll; ++u) {
+          const int q = q0 + u * wave;
+          const bool ok = u < unroll && q < ntap;
+          const int qq = ok ? q : 0;
+          const int yl = int(div_magic(unsigned(qq), unsigned(wdt), mg));
+          tx[u] = xa + (qq - yl * wdt);
+          ty[u] = ya + yl;
+#pragma unroll
+          for (int v = 0; v < N; ++v)
+            cv[u][v] = use[v] && ok ? slice[v][ty[u] * g.sx + tx[u]] : make_float2(0.0f, 0.0f);
+        }
         if (grad) {
+          // one tap (u = 0) with its phase screen
+          const int X = tx[0], Y = ty[0];
           const double phX = phi0x + X * dphix;
           const double phY = phi0y + Y * dphiy;
-          scr = narrow(cphase(phX + phY));
+          const float2 scr = narrow(cphase(phX + phY));
+          const int li = (gy + Y - y0) * tile + (gx + X - x0);
+          double sr = 0.0, si = 0.0;
+          for (int v = 0; v < N; ++v) {
+            if (!use[v]) continue;
+            float2 c = cv[0][v];
+            c.y *= imf;
+            c = cmul(c, scr);
+            const float2 pr = cmul(c, gv[v]);
+            sr += double(pr.x);
+            si += double(pr.y);
+          }
+          if constexpr (MODE == 0) {
+            __hip_atomic_fetch_add(acc64 + li, sr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(acc64 + TT + li, si, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          } else {
+            const float hr = float(sr), hi = float(si);
+            const float lr = float(sr - double(hr)), lim = float(si - double(hi));
+            pair_add<(N > 1)>(pairs + li, pairs[li], hr, lr);
+            pair_add<(N > 1)>(pairs + TT + li, pairs[TT + li], hi, lim);
+          }
+          continue;
         }
-        const int li = (gy + Y - y0) * tile + (gx + X - x0);
-        double sr = 0.0, si = 0.0;
-        for (int v = 0; v < N; ++v) {
-          if (!use[v]) continue;
-          float2 c = slice[v][Y * g.sx + X];
-          c.y *= imf;
-          if (grad) c = cmul(c, scr);
-          const float2 pr = cmul(c, gv[v]);
-          sr += double(pr.x);
-          si += double(pr.y);
-        }
-        if constexpr (MODE == 0) {
-          __hip_atomic_fetch_add(acc64 + li, sr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_fetch_add(acc64 + TT + li, si, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else {
-          // split the FP64 tap sum into float hi + lo (exact for N == 1)
-          const float hr = float(sr), hi = float(si);
-          const float lr = float(sr - double(hr)), lim = float(si - double(hi));
-          pair_add<(N > 1)>(pairs + li, pairs[li], hr, lr);
-          pair_add<(N > 1)>(pairs + TT + li, pairs[TT + li], hi, lim);
+#pragma unroll
+        for (int u = 0; u < tile_unroll; ++u) {
+          if (q0 + u * wave >= ntap) break;
+          const int li = (gy + ty[u] - y0) * tile + (gx + tx[u] - x0);
+          double sr = 0.0, si = 0.0;
+          for (int v = 0; v < N; ++v) {
+            if (!use[v]) continue;
+            float2 c = cv[u][v];
+            c.y *= imf;
+            const float2 pr = cmul(c, gv[v]);
+            sr += double(pr.x);
+            si += double(pr.y);
+          }
+          if constexpr (MODE == 0) {
+            __hip_atomic_fetch_add(acc64 + li, sr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(acc64 + TT + li, si, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          } else {
+            // split the FP64 tap sum into float hi + lo (exact for N == 1)
+            const float hr = float(sr), hi = float(si);
+            const float lr = float(sr - double(hr)), lim = float(si - double(hi));
+            pair_add<(N > 1)>(pairs + li, pairs[li], hr, lr);
+            pair_add<(N > 1)>(pairs + TT + li, pairs[TT + li], hi, lim);
+          }
         }
       }
     }
@@ -3741,9 +3829,11 @@ hipError_t launch_tile_stream(
   hipStream_t s, int npol, int mode, size_t nkeys, const Geometry& geo,
   const DeviceCF& cf, const Mueller& mu, const CallBuffers& cb, void* grid) {
   if (nkeys == 0) return hipSuccess;
+  const bool unr = size_t(cf.max_sx) * size_t(cf.max_sy) >= size_t(unroll_min_taps);
 #define HPGX_TILE2(NP, MD)                                                     \
   hipLaunchKernelGGL(                                                          \
-    (k_tile2<NP, MD>), dim3(unsigned(nkeys)), dim3(block), 0, s, geo,          \
+    unr ? (k_tile2<NP, MD, true>) : (k_tile2<NP, MD, false>),                  \
+    dim3(unsigned(nkeys)), dim3(block), 0, s, geo,                             \
     cf.groups, cf.taps, mu, cb.offsets,                                        \
     static_cast<const Entry<NP>*>(cb.stream_entries), cb.entry_vis,            \
     static_cast<const VisRec<NP>*>(cb.vis), grid)
