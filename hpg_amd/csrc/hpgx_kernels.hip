@@ -2655,13 +2655,89 @@ __device__ __forceinline__ void sub_gradient_entry(
   *a = t;
 }
 
+// The entry lists of tile key tkey's four quadrant keys: seg_pref[q][b] =
+// entries of quadrant q in the bin blocks before b (seg_pref[q][nbins] the
+// quadrant's total), seg_beg[q][b] = its first entry within block b.  GB
+// (global bins, grids of more keys than one workgroup's LDS histogram
+// holds): off_table is the exclusive scan of the quadrant keys' entry
+// counts, each key one contiguous range from gb_beg[q] (nbins = 1).  Ends
+// with a barrier; the block has at least four waves.
+template <int BLOCK, bool GB>
+__device__ __forceinline__ void sub_tables(
+  const unsigned* __restrict__ off_table, int nbins, unsigned ntk, unsigned tkey,
+  unsigned (*seg_pref)[sub_bins_max + 1], unsigned short (*seg_beg)[sub_bins_max],
+  unsigned* gb_beg) {
+  const unsigned tid = threadIdx.x;
+  if constexpr (GB) {
+    if (tid < 4) {
+      const unsigned s0 = off_table[4 * size_t(tkey) + tid];
+      const unsigned s1 = off_table[4 * size_t(tkey) + tid + 1];
+      gb_beg[tid] = s0;
+      seg_pref[tid][0] = 0u;
+      seg_pref[tid][1] = s1 - s0;
+    }
+    __syncthreads();
+  } else {
+    // segment tables of the four quadrant keys, then their exclusive scans
+    const size_t row = size_t(4) * ntk + 1;  // offset-table row
+    for (int i = int(tid); i < 4 * nbins; i += BLOCK) {
+      const int bb = i >> 2, qq = i & 3;
+      const unsigned* tab = off_table + size_t(bb) * row + 4 * size_t(tkey) + qq;
+      const unsigned s0 = tab[0], s1 = tab[1];
+      seg_pref[qq][bb] = s1 - s0;
+      seg_beg[qq][bb] = (unsigned short)s0;
+    }
+    __syncthreads();
+    const int w = int(tid) / wave;
+    const int lane = int(tid) % wave;
+    if (w < 4) {
+      unsigned* a = seg_pref[w];
+      unsigned v[4];
+      unsigned sum = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int idx = lane * 4 + k;
+        v[k] = idx < nbins ? a[idx] : 0u;
+        sum += v[k];
+      }
+      unsigned incl = sum;
+#pragma unroll
+      for (int d = 1; d < wave; d <<= 1) {
+        const unsigned t = __shfl_up(incl, d, wave);
+        if (lane >= d) incl += t;
+      }
+      unsigned run = incl - sum;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int idx = lane * 4 + k;
+        if (idx < nbins) a[idx] = run;
+        run += v[k];
+      }
+      if (lane == wave - 1) a[nbins] = incl;
+    }
+    __syncthreads();
+  }
+}
+
+// entry i of quadrant q's list (sub_tables): its index in the entry array
+template <bool GB>
+__device__ __forceinline__ unsigned sub_locate(
+  unsigned i, int q, int nbins, size_t block_cap, const unsigned (*seg_pref)[sub_bins_max + 1],
+  const unsigned short (*seg_beg)[sub_bins_max], const unsigned* gb_beg) {
+  if constexpr (GB) return gb_beg[q] + i;
+  int lo = 0, hi = nbins;  // seg_pref[q][lo] <= i < seg_pref[q][hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (seg_pref[q][mid] <= i) lo = mid; else hi = mid;
+  }
+  return unsigned(lo) * unsigned(block_cap) + seg_beg[q][lo] + (i - seg_pref[q][lo]);
+}
+
 // PX: region pixels per lane; each wave's region is padded to PX * 64
 // pixels (>= (16 + hx) * (16 + hy)), so a lane folds pixels lane + 64 k with
 // no bounds test.  UNI: every CF group has the same support, whose lane mask
-// then holds for the whole call.  PK: see sub_pk_flip.  GB (global bins,
-// grids of more keys than one workgroup's LDS histogram holds): off_table
-// is the exclusive scan of the quadrant keys' entry counts (k_count and
-// k_emit with quad_keys), each key's entries one contiguous range.
+// then holds for the whole call.  PK: see sub_pk_flip.  GB: global bins
+// (sub_tables).
 template <int N, int PX, bool UNI, bool PK, bool GB>
 __global__ __launch_bounds__(sub_block) __attribute__((amdgpu_waves_per_eu(N == 1 ? 8 : 4)))
 void k_sub(
@@ -2684,60 +2760,13 @@ void k_sub(
   const int Q = QX * QY;
   const unsigned ntk = unsigned(geo.ntx) * geo.nty * geo.nr * geo.nc;
   const unsigned tkey = xcd_key(blockIdx.x, gridDim.x);  // one block per tile key
-  const size_t row = size_t(4) * ntk + 1;                // offset-table row
   const unsigned tid = threadIdx.x;
   const int w = int(tid) / wave;
   const int lane = int(tid) % wave;
   const int q = w & 3;
   const int half = w >> 2;
 
-  if constexpr (GB) {
-    // one range per quadrant key: seg_pref[q][nbins = 1] its length
-    if (tid < 4) {
-      const unsigned s0 = off_table[4 * size_t(tkey) + tid];
-      const unsigned s1 = off_table[4 * size_t(tkey) + tid + 1];
-      gb_beg[tid] = s0;
-      seg_pref[tid][0] = 0u;
-      seg_pref[tid][1] = s1 - s0;
-    }
-    __syncthreads();
-  } else {
-  // segment tables of the four quadrant keys, then their exclusive scans
-  for (int i = int(tid); i < 4 * nbins; i += sub_block) {
-    const int bb = i >> 2, qq = i & 3;
-    const unsigned* tab = off_table + size_t(bb) * row + 4 * size_t(tkey) + qq;
-    const unsigned s0 = tab[0], s1 = tab[1];
-    seg_pref[qq][bb] = s1 - s0;
-    seg_beg[qq][bb] = (unsigned short)s0;
-  }
-  __syncthreads();
-  if (w < 4) {
-    unsigned* a = seg_pref[w];
-    unsigned v[4];
-    unsigned sum = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int idx = lane * 4 + k;
-      v[k] = idx < nbins ? a[idx] : 0u;
-      sum += v[k];
-    }
-    unsigned incl = sum;
-#pragma unroll
-    for (int d = 1; d < wave; d <<= 1) {
-      const unsigned t = __shfl_up(incl, d, wave);
-      if (lane >= d) incl += t;
-    }
-    unsigned run = incl - sum;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int idx = lane * 4 + k;
-      if (idx < nbins) a[idx] = run;
-      run += v[k];
-    }
-    if (lane == wave - 1) a[nbins] = incl;
-  }
-  __syncthreads();
-  }
+  sub_tables<sub_block, GB>(off_table, nbins, ntk, tkey, seg_pref, seg_beg, gb_beg);
   if (seg_pref[0][nbins] + seg_pref[1][nbins] + seg_pref[2][nbins] + seg_pref[3][nbins] == 0u) {
     if (tid == 0) halo_set[tkey] = 0u;
     return;
@@ -2806,13 +2835,7 @@ void k_sub(
   // costs outweigh its balance)
   const unsigned total = seg_pref[q][nbins];
   auto locate = [&](unsigned i) {
-    if constexpr (GB) return gb_beg[q] + i;
-    int lo = 0, hi = nbins;  // seg_pref[q][lo] <= i < seg_pref[q][hi]
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (seg_pref[q][mid] <= i) lo = mid; else hi = mid;
-    }
-    return unsigned(lo) * unsigned(block_cap) + seg_beg[q][lo] + (i - seg_pref[q][lo]);
+    return sub_locate<GB>(i, q, nbins, block_cap, seg_pref, seg_beg, gb_beg);
   };
   const int lane_x = lane & 7;
   const int lane_y = sub_lane_y(lane);
@@ -2963,8 +2986,9 @@ void k_sub(
         flip = aw & 0x80000000u;
         addr = aw & 0x7fffffffu;
       }
+      // UNI: the slot loops run on the support's lanes only
       bool on;
-      if constexpr (UNI) on = uni_ok; else on = __builtin_amdgcn_inverse_ballot_w64(ring_lanes[r]);
+      if constexpr (UNI) on = true; else on = __builtin_amdgcn_inverse_ballot_w64(ring_lanes[r]);
       float h_r = 0.0f, h_i = 0.0f;
       double sr = 0.0, si = 0.0;
 #pragma unroll
@@ -3007,7 +3031,7 @@ void k_sub(
 #pragma unroll
     for (int hb = 0; hb < 2; ++hb) {
       if (unsigned(hb * 32) >= rmax) continue;
-      if (hb * 32 < nslots) {
+      if (hb * 32 < nslots && uni_ok) {
 #pragma unroll
         for (int j = 0; j < sub_ring; ++j) fetch(hb * 32 + j);
 #pragma unroll

@@ -213,6 +213,43 @@ def test_quadrant_kernels_vs_oracle(hpg, oracle_mod, monkeypatch, sub, case):
     np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
 
 
+ONE_SUPPORT_CASES = [
+    # grid_size, radii (one support for every group), ncube, os, records
+    ((40, 36, 1, 1), [0], 1, 10, 6000),
+    ((64, 64, 1, 2), [1, 1], 2, 6, 9000),
+    ((100, 70, 1, 1), [2], 1, 8, 4000),
+    ((96, 96, 1, 3), [(3, 2), (3, 2)], 2, 5, 12000),
+    ((130, 97, 1, 1), [(1, 3)], 1, 20, 3000),
+    ((33, 65, 1, 1), [3], 1, 9, 7000),
+]
+
+
+@pytest.mark.parametrize("case", range(len(ONE_SUPPORT_CASES)))
+def test_quadrant_kernel_one_support_vs_oracle(hpg, oracle_mod, case):
+    """The quadrant-key kernel (k_sub) with one polarization and one support
+    of 1 to 7 pixels a side for every CF group (its lanes of the support
+    fixed for the call) against the oracle: quadrant lists of several
+    64-entry batches, phase gradients on a third of the records, records off
+    the grid, ragged grid edges: e_max, e_rms <= 1e-6 per plane and the
+    weights to 1e-12."""
+    from hpg_amd import workloads
+    gsz, radii, nc, os_, nrec = ONE_SUPPORT_CASES[case]
+    cf = workloads.random_cf(os_, radii, nmueller=1, ncube=nc, seed=230 + case)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (0.9, 1.1), [[0]], cf, 1)
+    rec = workloads.random_visibilities(nrec, 1, gsz, (0.9, 1.1), cf, seed=240 + case,
+                                        phase=True, margin=-3, gradient=True)
+    rec["cf_phase_gradient"][1::3] = 0.0
+    rec["cf_phase_gradient"][2::3] = 0.0
+    gs.grid_visibilities(hpg.Device.Serial, rec, True)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    got, exp = gs.grid_values(), o.grid_values()
+    for c in range(gsz[3]):
+        assert np.abs(exp[c, 0]).max() > 0
+        e_max, e_rms = rel_errors(got[c, 0], exp[c, 0])
+        assert e_max <= 1e-6 and e_rms <= 1e-6, (c, e_max, e_rms)
+    np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
+
+
 GLOBAL_SUB_RADII = [[3], [1, (3, 2), 2], [1]]
 
 
