@@ -919,7 +919,8 @@ constexpr int bin_keys_max = 8192;        // (plane, tile) keys of the block-loc
 constexpr int bin_quad_keys_max = 16384;  // quadrant keys (4 per tile key): LDS histogram
 constexpr int bin_planes_max = 1024;
 constexpr int tile_bins_max = 2048;  // bin blocks per pass (LDS of k_tile3)
-constexpr int bin_stage_bytes = 32768;  // LDS staging of k_bin's haloed entries
+constexpr int bin_stage_bytes = 65536;  // LDS staging of k_bin's haloed entries (one
+                                        // chunk for a block of one-polarization entries)
 
 // block-wide exclusive scan of a[0, n) in LDS (n <= bin_quad_keys_max), returns
 // the total; `part` holds blockDim.x / wave + 1 values
@@ -941,14 +942,18 @@ __device__ unsigned block_exclusive_scan(unsigned* a, int n, unsigned* part) {
   const int w = threadIdx.x / wave;
   if (lane == wave - 1) part[w] = v;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned acc = 0;
-    for (int k = 0; k < nt / wave; ++k) {
-      const unsigned t = part[k];
-      part[k] = acc;
-      acc += t;
+  if (w == 0) {
+    // exclusive scan of the wave totals (at most 64 waves) by one wave
+    const int nw = nt / wave;
+    const unsigned t = lane < nw ? part[lane] : 0u;
+    unsigned incl = t;
+#pragma unroll
+    for (int d = 1; d < wave; d <<= 1) {
+      const unsigned u = __shfl_up(incl, d, wave);
+      if (lane >= d) incl += u;
     }
-    part[nt / wave] = acc;
+    if (lane < nw) part[lane] = incl - t;
+    if (lane == nw - 1) part[nw] = incl;
   }
   __syncthreads();
   unsigned run = part[w] + v - sum;  // exclusive prefix of this thread
@@ -1189,7 +1194,8 @@ __global__ __launch_bounds__(bin_threads) void k_bin(
       const int lk = __shfl(key, leader, wave);
       double part_w[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) part_w[k] = wave_sum(key == lk ? wacc[k] : 0.0);
+      for (int k = 0; k < 4; ++k)
+        part_w[k] = k < geo.nr ? wave_sum(key == lk ? wacc[k] : 0.0) : 0.0;
       if (lane == leader) flush_weights(lk, part_w);
       if (key == lk) key = -1;
     }
