@@ -431,6 +431,16 @@ int ensure_call_buffers(hpgx_state_t st) {
       HPGX_HIP(st, dev_alloc(cb.tally_keys, need * 2 * sizeof(unsigned)));
       cb.tally_capacity = need;
     }
+    // the wave-per-record degridder's record order, once a model is set
+    if (st->model && hpgx::degrid_sorts(st->cf) && st->vis_capacity > cb.order_capacity) {
+      dev_free(cb.degrid_order);
+      dev_free(cb.order_temp);
+      cb.order_capacity = 0;
+      cb.order_temp_bytes = hpgx::degrid_order_temp_bytes(st->vis_capacity);
+      HPGX_HIP(st, dev_alloc(cb.degrid_order, 4 * st->vis_capacity * sizeof(unsigned)));
+      HPGX_HIP(st, dev_alloc(cb.order_temp, cb.order_temp_bytes));
+      cb.order_capacity = st->vis_capacity;
+    }
     if (need > cb.entries_capacity) {
       dev_free(cb.stream_entries);
       dev_free(cb.entry_vis);
@@ -486,6 +496,8 @@ void free_state(hpgx_state_t st) {
   dev_free(cb.halo_set);
   dev_free(cb.degrid_acc);
   dev_free(cb.scan_temp);
+  dev_free(cb.degrid_order);
+  dev_free(cb.order_temp);
   dev_free(cb.invalid_count);
   if (st->plan_z2z.valid) hipfftDestroy(st->plan_z2z.h);
   if (st->plan_c2c.valid) hipfftDestroy(st->plan_c2c.h);
@@ -1296,7 +1308,7 @@ int hpgx_set_convolution_function(
   const int nf = 2 * F + 1;
   std::vector<GroupInfo> groups(num_groups);
   long long toff = 0, woff = 0;
-  int max_sx = 1, max_sy = 1;
+  int max_sx = 1, max_sy = 1, max_ncube = 1;
   size_t max_stage = 0;
   for (unsigned g = 0; g < num_groups; ++g) {
     const unsigned* e = extents + 4 * g;
@@ -1313,6 +1325,7 @@ int hpgx_set_convolution_function(
     woff += slices;
     max_sx = std::max(max_sx, gi.sx);
     max_sy = std::max(max_sy, gi.sy);
+    max_ncube = std::max(max_ncube, gi.ncube);
     max_stage = std::max(max_stage, size_t(e[0]) * e[1] * e[2] * e[3]);
   }
   if (int rc = ensure_cf_capacity(st, size_t(toff), size_t(woff))) return rc;
@@ -1346,6 +1359,7 @@ int hpgx_set_convolution_function(
   st->cf.ngroups = int(num_groups);
   st->cf.max_sx = max_sx;
   st->cf.max_sy = max_sy;
+  st->cf.max_ncube = max_ncube;
   return ensure_call_buffers(st);
 }
 

@@ -84,6 +84,11 @@ struct CallBuffers {
   void* tally_entries;   // launch_tally: entries in record order (Entry4<N>)
   void* tally_keys;      // launch_tally: (key, place within the key) per slot
   size_t tally_capacity; // slots of both
+  unsigned* degrid_order; // launch_degrid (wave form): keys, sorted keys, record
+                          // indexes, sorted indexes, order_capacity each
+  size_t order_capacity;
+  void* order_temp;       // the sort's temporary storage
+  size_t order_temp_bytes;
 };
 
 struct DeviceCF {
@@ -92,6 +97,7 @@ struct DeviceCF {
   double2* wsum;
   int os, F, nf, ngroups;
   int max_sx, max_sy;
+  int max_ncube;  // CF cubes of the group with the most
 };
 
 // ---- launchers (hpgx_kernels.hip) -------------------------------------
@@ -203,6 +209,12 @@ hipError_t launch_degrid_finish(
 // records are only read, from `records` when given (else cb.vis; `records`
 // requires supports of at most degrid_lane_max_taps() taps)
 size_t degrid_lane_max_taps();
+// the wave-per-record degridder runs (supports above degrid_lane_max_taps()
+// taps) and can take its records in CF-slice order: with cb.degrid_order
+// for at least n records it sorts them by (CF group, cube, fine offset)
+// first, so that records reading the same CF slices run together
+bool degrid_sorts(const DeviceCF& cf);
+size_t degrid_order_temp_bytes(size_t n);
 hipError_t launch_degrid(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, const double2* model, bool do_grid, CallBuffers& cb,

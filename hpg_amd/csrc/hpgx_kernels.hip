@@ -3099,16 +3099,51 @@ void k_sub(
 
 // ---- degridding --------------------------------------------------------
 
+// The wave-per-record degridder's record order: key = the record's CF slice
+// (group, CF cube, fine offsets: grp * stride + (cube * nf + fxi) * nf +
+// fyi), kmax for records it skips (no CF, outside the grid).  Sorted by key,
+// the records that read the same 16 Mueller slices run together and find
+// them in L2 (a record's taps of all Mueller elements are read once per
+// slice set and batch rather than once per record).
+template <int N>
+__global__ void k_degrid_keys(
+  size_t n, Geometry geo, const GroupInfo* __restrict__ groups,
+  const VisRec<N>* __restrict__ recs, unsigned stride, unsigned kmax,
+  unsigned* __restrict__ keys, unsigned* __restrict__ idx) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const VisRec<N>& r = recs[i];
+  const unsigned grp = r.cf_index[1];
+  const unsigned cfcube = r.cf_index[0];
+  const unsigned gcube = r.grid_cube;
+  unsigned key = kmax;
+  if (grp < unsigned(geo.ngroups) && gcube < unsigned(geo.nc)) {
+    const GroupInfo g = groups[grp];
+    if (cfcube < unsigned(g.ncube)) {
+      const Coord c = vis_coord(r, geo, g);
+      if (c.inside)
+        key = grp * stride
+          + (cfcube * unsigned(geo.nf) + unsigned(c.fx + geo.F)) * unsigned(geo.nf)
+          + unsigned(c.fy + geo.F);
+    }
+  }
+  keys[i] = key;
+  idx[i] = unsigned(i);
+}
+
+// order: the records' processing order (k_degrid_keys, sorted), or null
 template <int N>
 __global__ __launch_bounds__(block) void k_degrid(
   size_t n, Geometry geo, const GroupInfo* __restrict__ groups,
   const float2* __restrict__ taps, const double2* __restrict__ wsum, Mueller mu,
   const double2* __restrict__ model, int do_grid, VisRec<N>* __restrict__ recs,
-  float2* __restrict__ gvis, double2* __restrict__ pred64) {
+  float2* __restrict__ gvis, double2* __restrict__ pred64,
+  const unsigned* __restrict__ order) {
 
-  const size_t i = blockIdx.x * (size_t)(block / wave) + threadIdx.x / wave;
+  const size_t wi = blockIdx.x * (size_t)(block / wave) + threadIdx.x / wave;
   const int lane = threadIdx.x % wave;
-  if (i >= n) return;
+  if (wi >= n) return;
+  const size_t i = order ? size_t(order[wi]) : wi;
   VisRec<N>& r = recs[i];
   const unsigned grp = r.cf_index[1];
   const unsigned cfcube = r.cf_index[0];
@@ -4200,6 +4235,19 @@ hipError_t launch_degrid_finish(
   return hipGetLastError();
 }
 
+bool degrid_sorts(const DeviceCF& cf) {
+  return size_t(cf.max_sx) * size_t(cf.max_sy) > size_t(degrid_lane_taps);
+}
+
+size_t degrid_order_temp_bytes(size_t n) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(
+    nullptr, bytes, static_cast<unsigned*>(nullptr), static_cast<unsigned*>(nullptr),
+    static_cast<unsigned*>(nullptr), static_cast<unsigned*>(nullptr),
+    int(std::min<size_t>(n, 0x7fffffffull)), 0, 32);
+  return bytes;
+}
+
 hipError_t launch_degrid(
   hipStream_t s, int npol, size_t n, const Geometry& geo, const DeviceCF& cf,
   const Mueller& mu, const double2* model, bool do_grid, CallBuffers& cb,
@@ -4225,12 +4273,45 @@ hipError_t launch_degrid(
   }
   if (records) return hipErrorInvalidValue;  // the wave form reads cb.vis
   // (callers stage records for supports above degrid_lane_max_taps())
+  // the records in CF-slice order where the order buffers hold the batch
+  // and the keys fit 32 bits
+  const unsigned* order = nullptr;
+  const unsigned long long stride = (unsigned long long)std::max(cf.max_ncube, 1) * cf.nf * cf.nf;
+  const unsigned long long kmax = stride * (unsigned long long)cf.ngroups;
+  if (model && cb.degrid_order && cb.order_capacity >= n && n <= 0x7fffffffull
+      && kmax > 0 && kmax < 0xffffffffull) {
+    unsigned* keys = cb.degrid_order;
+    unsigned* keys_sorted = keys + cb.order_capacity;
+    unsigned* idx = keys + 2 * cb.order_capacity;
+    unsigned* idx_sorted = keys + 3 * cb.order_capacity;
+#define HPGX_DKEYS(NP)                                                         \
+  hipLaunchKernelGGL(                                                          \
+    k_degrid_keys<NP>, dim3(blocks_for(n)), dim3(block), 0, s, n, geo,         \
+    cf.groups, static_cast<const VisRec<NP>*>(cb.vis), unsigned(stride),       \
+    unsigned(kmax), keys, idx)
+    switch (npol) {
+    case 1: HPGX_DKEYS(1); break;
+    case 2: HPGX_DKEYS(2); break;
+    case 3: HPGX_DKEYS(3); break;
+    case 4: HPGX_DKEYS(4); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef HPGX_DKEYS
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int bits = 32 - __builtin_clz(unsigned(kmax));
+    size_t bytes = cb.order_temp_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(cb.order_temp, bytes, keys, keys_sorted, idx,
+                                           idx_sorted, int(n), 0, bits, s);
+    if (e != hipSuccess) return e;
+    order = idx_sorted;
+  }
   const unsigned nb = unsigned((n + (block / wave) - 1) / (block / wave));
 #define HPGX_DEGRID(NP)                                                        \
   hipLaunchKernelGGL(                                                          \
     k_degrid<NP>, dim3(nb), dim3(block), 0, s, n, geo, cf.groups, cf.taps,     \
     cf.wsum, mu, model, int(do_grid), static_cast<VisRec<NP>*>(cb.vis),        \
-    cb.gvis, pred64)
+    cb.gvis, pred64, order)
   switch (npol) {
   case 1: HPGX_DEGRID(1); break;
   case 2: HPGX_DEGRID(2); break;
