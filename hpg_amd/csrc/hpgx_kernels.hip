@@ -2564,8 +2564,8 @@ __global__ __launch_bounds__(quad_block) void k_quad(
 // (an entry's taps are distinct pixels, the region is the wave's own and a
 // wave's LDS operations execute in order).  The batches run in rounds,
 // in step across the workgroup (two halves of 32 slots, each followed by
-// its entries with a phase gradient); after each round every wave folds
-// (between two barriers): wave q + 4 h holds the
+// its entries with a phase gradient); after every second round every wave
+// folds (between two barriers): wave q + 4 h holds the
 // (hi, lo) pairs of half h of quadrant q's region pixels (lane l: pixels
 // l + 64 k), adds both of the quadrant's regions' FP32 sums of those pixels
 // to its pairs by two-sum and clears them.  So the products of a pixel are
@@ -2577,7 +2577,8 @@ __global__ __launch_bounds__(quad_block) void k_quad(
 constexpr int sub_waves = 8;
 constexpr int sub_block = sub_waves * wave;
 constexpr int sub_bins_max = 256;  // bin blocks per pass (LDS segment tables)
-constexpr int sub_fold = 64;       // entries of a region between folds (one batch)
+constexpr int sub_fold_rounds = 2; // rounds between folds
+constexpr int sub_fold = 64 * sub_fold_rounds;  // entries of a region between folds
 constexpr int sub_ring = 8;        // slots whose CF taps load ahead
 constexpr int sub_flush = 3;       // tile + halo pixels per thread in the flush
 
@@ -3031,9 +3032,10 @@ void k_sub(
       }
     };
     // two halves of 32 slots, each followed by the half's entries with a
-    // phase gradient; the shared fold after the round's last half, so at
-    // most sub_fold entries reach a region between folds.  A half without
-    // entries in any wave of the workgroup is skipped
+    // phase gradient; the shared fold after the last half of every
+    // sub_fold_rounds-th round and of the last, so at most sub_fold entries
+    // reach a region between folds.  A half without entries in any wave of
+    // the workgroup is skipped
 #pragma unroll
     for (int hb = 0; hb < 2; ++hb) {
       if (unsigned(hb * 32) >= rmax) continue;
@@ -3075,7 +3077,7 @@ void k_sub(
           | (gpos_w ? quad_pos_w : 0u);
         sub_gradient_entry<N>(w0, sl, gv, entry_vis[ej], ge.h.pos, geo, taps, recs, QX, region);
       }
-      if (last_half) {
+      if (last_half && ((rd + 1u) % unsigned(sub_fold_rounds) == 0u || rd + 1u == nrounds)) {
         __syncthreads();
         fold();
         __syncthreads();
