@@ -2577,7 +2577,7 @@ __global__ __launch_bounds__(quad_block) void k_quad(
 constexpr int sub_waves = 8;
 constexpr int sub_block = sub_waves * wave;
 constexpr int sub_bins_max = 256;  // bin blocks per pass (LDS segment tables)
-constexpr int sub_fold_rounds = 2; // rounds between folds
+constexpr int sub_fold_rounds = 4; // rounds between folds
 constexpr int sub_fold = 64 * sub_fold_rounds;  // entries of a region between folds
 constexpr int sub_ring = 8;        // slots whose CF taps load ahead
 constexpr int sub_flush = 3;       // tile + halo pixels per thread in the flush
