@@ -250,6 +250,42 @@ def test_quadrant_kernel_one_support_vs_oracle(hpg, oracle_mod, case):
     np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
 
 
+DENSE_CASES = [
+    # npol, M, Mc, nmueller
+    (1, [[0]], [[0]], 1),
+    (2, [[0, 1], [2, 3]], [[1, 0], [3, 2]], 4),
+    (4, [[0, 1, 2, 3], [4, 5, 6, 7], [8, 9, 10, 11], [12, 13, 14, 15]],
+     [[0, 1, 2, 3], [4, 5, 6, 7], [8, 9, 10, 11], [12, 13, 14, 15]], 16),
+]
+
+
+@pytest.mark.parametrize("case", range(len(DENSE_CASES)))
+def test_quadrant_kernel_dense_folds_vs_oracle(hpg, oracle_mod, case):
+    """The quadrant-key kernel on quadrants of over a thousand entries each
+    (24000 records on a 64 x 64 grid: about ten rounds of batches, so the
+    shared fold runs inside the round loop as well as after it), one, two
+    and four polarizations, phase gradients on a third of the records:
+    e_max, e_rms <= 1e-6 per plane, weights to 1e-12."""
+    from hpg_amd import workloads
+    npol, M, Mc, nm = DENSE_CASES[case]
+    gsz = (64, 64, len(M), 1)
+    cf = workloads.random_cf(8, [3, (2, 3)], nmueller=nm, ncube=1, seed=260 + case)
+    gs, o = make_pair(hpg, oracle_mod, gsz, (1.0, 1.0), M, cf, 1, Mc)
+    rec = workloads.random_visibilities(24000, npol, gsz, (1.0, 1.0), cf, seed=270 + case,
+                                        phase=True, margin=4, gradient=True)
+    rec["cf_phase_gradient"][1::3] = 0.0
+    rec["cf_phase_gradient"][2::3] = 0.0
+    print(f"dense quadrants npol {npol}", flush=True)
+    gs.grid_visibilities(hpg.Device.Serial, rec, True)
+    o.grid_visibilities(oracle_view(rec, oracle_mod), True)
+    got, exp = gs.grid_values(), o.grid_values()
+    for r in range(gsz[2]):
+        assert np.abs(exp[0, r]).max() > 0
+        e_max, e_rms = rel_errors(got[0, r], exp[0, r])
+        assert e_max <= 1e-6 and e_rms <= 1e-6, (r, e_max, e_rms)
+    np.testing.assert_allclose(gs.grid_weights(), o.grid_weights(), rtol=1e-12)
+
+
 GLOBAL_SUB_RADII = [[3], [1, (3, 2), 2], [1]]
 
 
