@@ -21,6 +21,9 @@ runs of the same workload, summarized into profiles/:
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d F -- python3 bench.py --pmc-workload
     rocprofv3 --pmc WRITE_SIZE --output-format csv -d W -- python3 bench.py --pmc-workload
     python3 bench.py --pmc-summary F W > profiles/traffic_grid_1024sq_1e6vis.json
+
+`--configs-only` runs only the multi-polarization workloads' legs (the
+`configs` figures), for a kernel trace of them.
 """
 import argparse
 import json
@@ -568,6 +571,9 @@ def main():
                     help="skip the multi-polarization workloads' legs")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run the program profiled by the counter runs and exit")
+    ap.add_argument("--configs-only", action="store_true",
+                    help="run only the multi-polarization workloads' legs (one JSON "
+                         "line of their figures) and exit")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarize the two counter runs' CSVs and exit")
     args = ap.parse_args()
@@ -585,6 +591,9 @@ def main():
     from hpg_amd import workloads
 
     hpg.initialize(0)
+    if args.configs_only:
+        print(json.dumps({"configs": config_legs(hpg)}), flush=True)
+        return
     cf = workloads.gaussian_cf(CF_RADIUS, CF_OS, 1.5)
     records = workloads.config1_visibilities(NUM_VIS, GRID)
     dev_records = torch.from_numpy(records.view(np.uint8).copy()).cuda()

@@ -207,20 +207,29 @@ __global__ void k_cf_relayout(
   taps[gi.tap_offset + o] = v;
 }
 
-__global__ void k_cf_wsum(
+// the FP64 sum of each CF slice's taps (the weights' normalization), one
+// wave per slice: lane l sums taps l, l + 64, ... (coalesced reads), then a
+// fixed-order wave reduction.  FP64 sums of FP32 taps: exact while the
+// taps' exponents span fewer than about 29 bits, else within a few FP64
+// ulps of any other order
+constexpr int wsum_block = 256;
+__global__ __launch_bounds__(wsum_block) void k_cf_wsum(
   GroupInfo gi, int nf, const float2* __restrict__ taps,
   double2* __restrict__ wsum, long long nslices) {
-  const long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (s >= nslices) return;
-  const float2* t = taps + gi.tap_offset + s * gi.sx * gi.sy;
+  const long long s = blockIdx.x * (long long)(wsum_block / wave) + threadIdx.x / wave;
+  const int lane = int(threadIdx.x % wave);
+  if (s >= nslices) return;  // the whole wave
+  const int ntap = gi.sx * gi.sy;
+  const float2* t = taps + gi.tap_offset + s * ntap;
   double re = 0.0, im = 0.0;
-  for (int X = 0; X < gi.sx; ++X)
-    for (int Y = 0; Y < gi.sy; ++Y) {
-      const float2 c = t[Y * gi.sx + X];
-      re += double(c.x);
-      im += double(c.y);
-    }
-  wsum[gi.wsum_offset + s] = make_double2(re, im);
+  for (int k = lane; k < ntap; k += wave) {
+    const float2 c = t[k];
+    re += double(c.x);
+    im += double(c.y);
+  }
+  re = wave_sum(re);
+  im = wave_sum(im);
+  if (lane == 0) wsum[gi.wsum_offset + s] = make_double2(re, im);
 }
 
 // ---- gridding ----------------------------------------------------------
@@ -3769,8 +3778,8 @@ hipError_t launch_cf_wsum(
   const long long ns = (long long)gi.ncube * gi.nmueller * nf * nf;
   if (ns == 0) return hipSuccess;
   hipLaunchKernelGGL(
-    k_cf_wsum, dim3(blocks_for(size_t(ns), 64)), dim3(64), 0, s, gi, nf, taps,
-    wsum, ns);
+    k_cf_wsum, dim3(blocks_for(size_t(ns), wsum_block / wave)), dim3(wsum_block), 0, s, gi,
+    nf, taps, wsum, ns);
   return hipGetLastError();
 }
 
