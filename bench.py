@@ -246,16 +246,21 @@ def config_legs(hpg):
     the first 1e6 records.  FLOPs: 8 per (tap, vpol) gridded and 10
     degridded, over records whose support lies inside the grid; bytes: the
     records, the CF in the reference's layout, the grid read and written,
-    the weights, the model read when degridding."""
+    the weights, the model read when degridding.  gather_tbps: the CF and
+    model bytes the kernels gather per second (from L2 or HBM; per tap and
+    Mueller term 8 bytes of CF, gridding and degridding each, and 16 bytes
+    of model per tap and grid row degridded), the traffic that bounds
+    these workloads, as against the compulsory bytes of hbm_frac."""
     import torch
     from hpg_amd import workloads
     out = {}
     M = workloads.MUELLER_FULL
 
-    def fractions(flops, nbytes, sec):
+    def fractions(flops, nbytes, sec, gathered):
         return {"fp32_frac": flops / sec / (FP32_PEAK_TFLOPS * 1e12),
                 "hbm_frac": nbytes / sec / (HBM_PEAK_GBS * 1e9),
-                "tflops": flops / sec / 1e12}
+                "tflops": flops / sec / 1e12,
+                "gather_tbps": gathered / sec / 1e12}
 
     # full-pol W-projection
     gsz = (4096, 4096, 4, 8)
@@ -284,7 +289,8 @@ def config_legs(hpg):
     nbytes = n * 120 + _cf_bytes(radii, 20, 16) + 3 * grid_b + 2 * 8 * 32
     out["fullpol_wproj_4096sq_5e7"] = dict(
         call="degrid_grid_visibilities (FP64 grid, model set)", records=n,
-        device_s=sec, vis_per_s=4 * n / sec, **fractions(flops, nbytes, sec))
+        device_s=sec, vis_per_s=4 * n / sec,
+        **fractions(flops, nbytes, sec, s2 * (16 * 8 * 2 + 4 * 16)))
     gs.release()
     del rec, fields
     torch.cuda.empty_cache()
@@ -316,11 +322,11 @@ def config_legs(hpg):
         forward=dict(call="degrid_forward (FP64 predictions)", records=n, device_s=fwd,
                      vis_per_s=4 * n / fwd,
                      **fractions(s2 * 4 * 10, n * (120 + 64) + _cf_bytes(radii, 16, 4)
-                                 + grid_b, fwd)),
+                                 + grid_b, fwd, s2 * (4 * 8 + 4 * 16))),
         adjoint=dict(call="degrid_adjoint (FP64 grid)", records=n, device_s=adj,
                      vis_per_s=4 * n / adj,
                      **fractions(s2 * 4 * 8, n * (120 + 64) + _cf_bytes(radii, 16, 4)
-                                 + 2 * grid_b, adj)))
+                                 + 2 * grid_b, adj, s2 * 4 * 8)))
     gs.release()
     del rec, resid
     torch.cuda.empty_cache()
@@ -353,7 +359,8 @@ def config_legs(hpg):
     nbytes = n * 120 + _cf_bytes(radii, 10, 16) + 2 * grid_b + 8192 * 8192 * 8 * 16
     out["awproj_8192sq_2ch_2.5e7"] = dict(
         call="degrid_grid_visibilities (FP32-compensated grid, FP64 model)", records=n,
-        device_s=sec, vis_per_s=4 * n / sec, **fractions(flops, nbytes, sec))
+        device_s=sec, vis_per_s=4 * n / sec,
+        **fractions(flops, nbytes, sec, s2 * (16 * 8 * 2 + 4 * 16)))
     gs.release()
     del rec
     torch.cuda.empty_cache()
