@@ -348,11 +348,16 @@ size_t entries_per_record(hpgx_state_t st) {
     * hpgx::tiles_per_footprint(std::max(1, st->cf.max_sx), std::max(1, st->cf.max_sy));
 }
 
-// device memory for bin entries (default 4 GiB; HPGX_ENTRY_BUDGET_MB)
+// device memory for bin entries (HPGX_ENTRY_BUDGET_MB; default 16 GiB,
+// under 6% of an MI355X's 288 GB: batches whose entries exceed it are
+// gridded in passes, each with its own binning and kernel tail -- 4 GiB
+// took the A/W-projection workload's call 12% longer, 64 GiB gained under
+// 1% more)
+constexpr long entry_budget_default_mb = 16384;
 size_t entry_budget_bytes() {
   const char* e = std::getenv("HPGX_ENTRY_BUDGET_MB");
-  const long mb = e ? std::atol(e) : 4096;
-  return size_t(mb > 0 ? mb : 4096) << 20;
+  const long mb = e ? std::atol(e) : entry_budget_default_mb;
+  return size_t(mb > 0 ? mb : entry_budget_default_mb) << 20;
 }
 
 // per-call buffers sized for max_batch records and the current CF supports
