@@ -2573,8 +2573,9 @@ __global__ __launch_bounds__(quad_block) void k_quad(
 // (an entry's taps are distinct pixels, the region is the wave's own and a
 // wave's LDS operations execute in order).  The batches run in rounds,
 // in step across the workgroup (two halves of 32 slots, each followed by
-// its entries with a phase gradient); after every second round every wave
-// folds (between two barriers): wave q + 4 h holds the
+// its entries with a phase gradient); after every sub_fold_rounds-th round
+// and the last, every wave folds (between two barriers): wave q + 4 h holds
+// the
 // (hi, lo) pairs of half h of quadrant q's region pixels (lane l: pixels
 // l + 64 k), adds both of the quadrant's regions' FP32 sums of those pixels
 // to its pairs by two-sum and clears them.  So the products of a pixel are
